@@ -1,0 +1,399 @@
+// Reference-compatible C-ABI (v1) for the D3Q19 BGK step on gfx950.
+//
+// Built twice: real_t = float (libbgk3d_gpu_sp.so) and, with -DBGK_DOUBLE,
+// real_t = double (libbgk3d_gpu_dp.so), matching the reference's SINGLE and
+// DOUBLE_P builds (SRC/col_MC_gpu.hpp:78-82, SRC/bcond_gpu.hpp:47-51).  The
+// symbols and argument lists are those the reference's Fortran binds
+// (SRC/col_MC.F90:101-123, SRC/bcond_gpu_mod.F90:30-77), so either library
+// links in place of the reference's col_MC_hip.o + bcond_hip.o.
+//
+// Ownership: all lattice memory belongs to the caller; these entry points
+// never allocate it, never free it and never keep its pointers across calls
+// (the caller swaps a and b every step).  The diagnostics keep a small cached
+// device scratch (partial sums) and pinned host buffer.
+#include "common.hpp"
+#include "d3q19.hpp"
+#include "reduce.hpp"
+#include "v1_kernels.hpp"
+
+#include "../../include/bgk3d_gpu.h"
+
+#include <cmath>
+
+#ifdef BGK_DOUBLE
+using real_t = double;
+#else
+using real_t = float;
+#endif
+
+namespace bgk {
+hipStream_t& abi_stream()
+{
+    static hipStream_t s = nullptr;  // null stream, as in the reference
+    return s;
+}
+}  // namespace bgk
+
+using namespace bgk;
+
+namespace {
+
+Lattice19<real_t> lat_from(real_t** a)
+{
+    Lattice19<real_t> L;
+    for (int d = 0; d < 19; ++d) L.p[d] = a[d];
+    return L;
+}
+
+Geometry geom(int l, int m, int n, int nx, int ny)
+{
+    Geometry g;
+    g.l = l;
+    g.m = m;
+    g.n = n;
+    g.nx = nx;
+    g.nxy = (long)nx * (long)ny;
+    return g;
+}
+
+constexpr int kFaceThreads = 256;
+
+template <typename K, typename... Args>
+void launch_face(K kern, long count, const char* name, Args... args)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(kern, dim3(ceil_div(count, kFaceThreads)), dim3(kFaceThreads), 0, abi_stream(), args...);
+    report_launch(name);
+}
+
+template <int FM>
+void launch_col(const Lattice19<real_t>& A, const Lattice19<real_t>& B, const Geometry& g, real_t omega,
+                real_t fgrad, real_t ux, real_t uy, real_t uz, const CollideConsts<real_t>& c)
+{
+    if (g.l <= 0 || g.m <= 0 || g.n <= 0) return;
+    dim3 grid(ceil_div(g.l, kColBX), ceil_div(g.m, kColBY), g.n);
+    dim3 block(kColBX, kColBY);
+    hipLaunchKernelGGL((col_mc_kernel<real_t, FM>), grid, block, 0, abi_stream(), A, B, g, omega, fgrad, ux, uy,
+                       uz, c);
+    report_launch("col_mc_kernel");
+}
+
+// Cached scratch for the reductions (per device; single host thread, as in
+// the reference's driver).
+struct Scratch {
+    double* dev = nullptr;
+    double* host = nullptr;
+    size_t cap = 0;
+};
+
+Scratch& scratch_for_device()
+{
+    static Scratch s[64];
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return s[dev & 63];
+}
+
+double* scratch(size_t doubles, double** host)
+{
+    Scratch& s = scratch_for_device();
+    if (s.cap < doubles) {
+        if (s.dev) (void)hipFree(s.dev);
+        if (s.host) (void)hipHostFree(s.host);
+        s.dev = nullptr;
+        s.host = nullptr;
+        s.cap = 0;
+        if (hipMalloc(&s.dev, doubles * sizeof(double)) != hipSuccess) return nullptr;
+        if (hipHostMalloc(&s.host, doubles * sizeof(double), hipHostMallocDefault) != hipSuccess) return nullptr;
+        s.cap = doubles;
+    }
+    *host = s.host;
+    return s.dev;
+}
+
+int fetch(const double* dev, double* host, size_t n)
+{
+    BGK_HIP_CHECK(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, abi_stream()));
+    BGK_HIP_CHECK(hipStreamSynchronize(abi_stream()));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void col_mc_gpu(const real_t* a01, const real_t* a02, const real_t* a03, const real_t* a04, const real_t* a05,
+                const real_t* a06, const real_t* a07, const real_t* a08, const real_t* a09, const real_t* a10,
+                const real_t* a11, const real_t* a12, const real_t* a13, const real_t* a14, const real_t* a15,
+                const real_t* a16, const real_t* a17, const real_t* a18, const real_t* a19, real_t* b01,
+                real_t* b02, real_t* b03, real_t* b04, real_t* b05, real_t* b06, real_t* b07, real_t* b08,
+                real_t* b09, real_t* b10, real_t* b11, real_t* b12, real_t* b13, real_t* b14, real_t* b15,
+                real_t* b16, real_t* b17, real_t* b18, int l, int m, int n, int nx, int ny, real_t omega,
+                real_t cte0, real_t cte1, real_t p0, real_t p1, real_t p2, real_t rf, real_t qf, real_t tre,
+                real_t forcex, real_t forcey, real_t forcez)
+{
+    const real_t* in[19] = {a01, a02, a03, a04, a05, a06, a07, a08, a09, a10,
+                            a11, a12, a13, a14, a15, a16, a17, a18, a19};
+    real_t* out[18] = {b01, b02, b03, b04, b05, b06, b07, b08, b09, b10, b11, b12, b13, b14, b15, b16, b17, b18};
+    Lattice19<real_t> A, B;
+    for (int d = 0; d < 19; ++d) A.p[d] = const_cast<real_t*>(in[d]);
+    for (int d = 0; d < 18; ++d) B.p[d] = out[d];
+    B.p[18] = A.p[18];  // direction 19 collides in place
+    CollideConsts<real_t> c{cte0, cte1, p0, p1, p2, rf, qf, tre};
+    launch_col<0>(A, B, geom(l, m, n, nx, ny), omega, real_t(0), forcex, forcey, forcez, c);
+}
+
+void col_mc_force_gpu(real_t** a, real_t** b, int l, int m, int n, int nx, int ny, real_t omega, real_t fgrad,
+                      int axis)
+{
+    Lattice19<real_t> A = lat_from(a), B = lat_from(b);
+    B.p[18] = A.p[18];
+    const Geometry g = geom(l, m, n, nx, ny);
+    const auto c = CollideConsts<real_t>::shifted();
+    switch (axis) {
+        case 1: launch_col<1>(A, B, g, omega, fgrad, 0, 0, 0, c); break;
+        case 2: launch_col<2>(A, B, g, omega, fgrad, 0, 0, 0, c); break;
+        case 3: launch_col<3>(A, B, g, omega, fgrad, 0, 0, 0, c); break;
+        default: launch_col<0>(A, B, g, omega, 0, 0, 0, 0, c); break;
+    }
+}
+
+void bcond_periodic_gpu(real_t** a, int l, int m, int n, int nx, int ny)
+{
+    const auto A = lat_from(a);
+    const auto g = geom(l, m, n, nx, ny);
+    launch_face(periodic_x<real_t>, (long)(m + 2) * (n + 2), "periodic_x", A, g);
+    launch_face(periodic_y<real_t>, (long)(l + 2) * (n + 2), "periodic_y", A, g);
+    launch_face(periodic_z<real_t>, (long)(l + 2) * (m + 2), "periodic_z", A, g);
+}
+
+void bcond_channel_gpu(real_t** a, int l, int m, int n, int nx, int ny)
+{
+    const auto A = lat_from(a);
+    const auto g = geom(l, m, n, nx, ny);
+    launch_face(periodic_x<real_t>, (long)(m + 2) * (n + 2), "periodic_x", A, g);
+    launch_face(periodic_y<real_t>, (long)(l + 2) * (n + 2), "periodic_y", A, g);
+    launch_face(walls_z<real_t>, (long)l * m, "walls_z", A, g, real_t(0), 0);
+}
+
+void bcond_driven_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t force)
+{
+    const auto A = lat_from(a);
+    const auto g = geom(l, m, n, nx, ny);
+    launch_face(driven_x<real_t>, (long)m * n, "driven_x", A, g);
+    launch_face(driven_y<real_t>, (long)l * n, "driven_y", A, g);
+    launch_face(walls_z<real_t>, (long)l * m, "walls_z", A, g, force, 1);
+}
+
+void bcond_inflow_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t u_inflow, real_t rf, real_t qf,
+                      real_t tre, real_t p1, real_t p2, real_t cte1)
+{
+    (void)tre;
+    const auto A = lat_from(a);
+    const auto g = geom(l, m, n, nx, ny);
+    launch_face(inflow_x<real_t>, (long)(m + 2) * (n + 2), "inflow_x", A, g, u_inflow, rf, qf, p1, p2, cte1);
+    launch_face(periodic_y<real_t>, (long)(l + 2) * (n + 2), "periodic_y", A, g);
+    launch_face(periodic_z<real_t>, (long)(l + 2) * (m + 2), "periodic_z", A, g);
+}
+
+void bcond_obs_gpu(real_t** a, const int* obs, int l, int m, int n, int nx, int ny, int imin, int imax, int jmin,
+                   int jmax, int kmin, int kmax, int obs_nx, int obs_ny)
+{
+    if (obs_nx != l || obs_ny != m) {
+        std::fprintf(stderr, "bgk3d_gpu: bcond_obs_gpu expects the (1:l,1:m,1:n) mask (got %d x %d)\n", obs_nx,
+                     obs_ny);
+        return;
+    }
+    if (imax < imin || jmax < jmin || kmax < kmin) return;  // no solid cell
+    const auto A = lat_from(a);
+    const auto g = geom(l, m, n, nx, ny);
+    launch_face(obstacle_bb<real_t>, (long)(imax - imin + 1) * (jmax - jmin + 1) * (kmax - kmin + 1),
+                "obstacle_bb", A, g, obs, imin, imax, jmin, jmax, kmin, kmax);
+}
+
+void gpu_device_sync(void)
+{
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "bgk3d_gpu: device sync failed: %s\n", hipGetErrorString(e));
+        std::fflush(stderr);
+        std::abort();
+    }
+}
+
+int bgk_set_stream(void* stream)
+{
+    abi_stream() = reinterpret_cast<hipStream_t>(stream);
+    return 0;
+}
+
+int diagno_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t cte1, double* out)
+{
+    if (l <= 0 || m <= 0 || n <= 0) return -1;
+    double* host = nullptr;
+    double* dev = scratch((size_t)(kRedBlocks + 1) * 5, &host);
+    if (!dev) return -2;
+    const long rows = (long)m * n;
+    const int nb = (int)(rows < kRedBlocks ? rows : kRedBlocks);
+    hipLaunchKernelGGL(diagno_partial<real_t>, dim3(nb), dim3(kRedThreads), 0, abi_stream(), lat_from(a),
+                       geom(l, m, n, nx, ny), cte1, dev + 5);
+    report_launch("diagno_partial");
+    hipLaunchKernelGGL(fold_partials<5>, dim3(1), dim3(kRedThreads), 0, abi_stream(), dev + 5, nb, dev);
+    report_launch("fold_partials");
+    if (int rc = fetch(dev, host, 5)) return rc;
+    // (x/float(l))/float(m)/float(n), SRC/diagno.F90:134-138
+    const double fl = (double)(float)l, fm = (double)(float)m, fn = (double)(float)n;
+    for (int q = 0; q < 5; ++q) out[q] = (host[q] / fl) / fm / fn;
+    return 0;
+}
+
+int varm_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t cte1, double* u_k, double* w_k, double* v_k,
+             double* den_j)
+{
+    if (l <= 0 || m <= 0 || n <= 0) return -1;
+    double* host = nullptr;
+    double* dev = scratch((size_t)3 * n + m, &host);
+    if (!dev) return -2;
+    const auto A = lat_from(a);
+    const auto g = geom(l, m, n, nx, ny);
+    hipLaunchKernelGGL(varm_planes_k<real_t>, dim3(n), dim3(kRedThreads), 0, abi_stream(), A, g, cte1, dev);
+    report_launch("varm_planes_k");
+    hipLaunchKernelGGL(varm_planes_j<real_t>, dim3(m), dim3(kRedThreads), 0, abi_stream(), A, g, cte1,
+                       dev + 3 * n);
+    report_launch("varm_planes_j");
+    if (int rc = fetch(dev, host, (size_t)3 * n + m)) return rc;
+    // rvol = 1.0/(float(l)) in default real (SRC/varm.F90:94)
+    const double rvol = (double)(real_t)(1.0f / (float)l);
+    for (int k = 0; k < n; ++k) {
+        u_k[k] = host[3 * k + 0] * rvol;
+        w_k[k] = host[3 * k + 1] * rvol;
+        v_k[k] = host[3 * k + 2] * rvol;
+    }
+    for (int j = 0; j < m; ++j) den_j[j] = host[3 * n + j] * rvol;
+    return 0;
+}
+
+int draglift_gpu(real_t** a, const int* obs, int l, int m, int n, int nx, int ny, int icoord, int jcoord,
+                 int kcoord, int radius, real_t u_inflow, double* out)
+{
+    // bounding box of SRC/draglift.F90:54-62 (delta = 3)
+    const int delta = 3;
+    const int is = icoord - radius - delta, ie = icoord + radius + delta;
+    const int js = jcoord - radius - delta, je = jcoord + radius + delta;
+    const int ks = kcoord - radius - delta, ke = kcoord + radius + delta;
+    // every obs(x + c) read must stay inside (1:l,1:m,1:n)
+    if (is < 2 || js < 2 || ks < 2 || ie > l - 1 || je > m - 1 || ke > n - 1) {
+        std::fprintf(stderr, "bgk3d_gpu: draglift box [%d:%d,%d:%d,%d:%d] leaves the mask\n", is, ie, js, je, ks, ke);
+        return -1;
+    }
+    double* host = nullptr;
+    double* dev = scratch((size_t)(kRedBlocks + 1) * 3, &host);
+    if (!dev) return -2;
+    const long cells = (long)(ie - is + 1) * (je - js + 1) * (ke - ks + 1);
+    const int nb = (int)((cells + kRedThreads - 1) / kRedThreads < kRedBlocks ? (cells + kRedThreads - 1) / kRedThreads
+                                                                                : kRedBlocks);
+    hipLaunchKernelGGL(draglift_partial<real_t>, dim3(nb), dim3(kRedThreads), 0, abi_stream(), lat_from(a),
+                       geom(l, m, n, nx, ny), obs, is, ie, js, je, ks, ke, dev + 3);
+    report_launch("draglift_partial");
+    hipLaunchKernelGGL(fold_partials<3>, dim3(1), dim3(kRedThreads), 0, abi_stream(), dev + 3, nb, dev);
+    report_launch("fold_partials");
+    if (int rc = fetch(dev, host, 3)) return rc;
+    // norm = uno/(u_inflow*u_inflow*radius) in the storage kind (:178)
+    const real_t norm = real_t(1) / (u_inflow * u_inflow * (real_t)radius);
+    for (int q = 0; q < 3; ++q) out[q] = host[q] * (double)norm;
+    return 0;
+}
+
+int prof_k_gpu(real_t** a, int l, int m, int n, int nx, int ny, int icoord, int jcoord, real_t cte1, real_t* out)
+{
+    if (icoord < 0 || icoord > l + 1 || jcoord < 0 || jcoord > m + 1 || n <= 0) return -1;
+    double* host = nullptr;
+    const size_t bytes = (size_t)4 * n * sizeof(real_t);
+    double* dev = scratch((bytes + sizeof(double) - 1) / sizeof(double), &host);
+    if (!dev) return -2;
+    real_t* d = reinterpret_cast<real_t*>(dev);
+    hipLaunchKernelGGL(prof_k_kernel<real_t>, dim3(ceil_div(n, 64)), dim3(64), 0, abi_stream(), lat_from(a),
+                       geom(l, m, n, nx, ny), icoord, jcoord, cte1, d);
+    report_launch("prof_k_kernel");
+    BGK_HIP_CHECK(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, abi_stream()));
+    BGK_HIP_CHECK(hipStreamSynchronize(abi_stream()));
+    return 0;
+}
+
+// Initial populations f = f_eq written into HOST arrays h[0..18] over every
+// cell including the halo (0..l+1, 0..m+1, 0..n+1), shifted storage
+// (SRC/init.F90:31-209; cte1 = 0 there).  mode 0: rest (all zero);
+// mode 1: 3-D Taylor-Green of SRC/init.F90:142-148.  Runs on the host as in
+// the reference, with the C library's sin/cos of the working kind, so the
+// state is bit-identical to the reference's.  Mixed-kind details kept:
+// pi is a default-real literal (:63) and the coordinates are formed in
+// double from (real(i,mykind)-0.5d0)/real(l,mykind) (:115-123).
+int bgk_init_host(real_t** h, int l, int m, int n, int nx, int ny, int mode, real_t u00)
+{
+    if (mode != 0 && mode != 1) return -1;
+    const size_t nxy = (size_t)nx * (size_t)ny;
+    const real_t uno = 1, due = 2, tre = 3, qua = 4, zero = 0, rf = 3, qf = real_t(1.5), cte1 = 0;
+    const real_t pi = (real_t)3.141592653589793238462643383279f;
+    const real_t w[19] = {W<real_t>::p2, W<real_t>::p2, W<real_t>::p2, W<real_t>::p2, W<real_t>::p1,
+                          W<real_t>::p1, W<real_t>::p2, W<real_t>::p1, W<real_t>::p2, W<real_t>::p2,
+                          W<real_t>::p2, W<real_t>::p2, W<real_t>::p2, W<real_t>::p1, W<real_t>::p1,
+                          W<real_t>::p2, W<real_t>::p1, W<real_t>::p2, W<real_t>::p0};
+    for (int k = 0; k <= n + 1; ++k) {
+        const real_t z = (real_t)(((double)(real_t)k - 0.5) / (double)(real_t)n);
+        for (int j = 0; j <= m + 1; ++j) {
+            const real_t y = (real_t)(((double)(real_t)j - 0.5) / (double)(real_t)m);
+            for (int i = 0; i <= l + 1; ++i) {
+                const real_t x = (real_t)(((double)(real_t)i - 0.5) / (double)(real_t)l);
+                real_t xj = zero, yj = zero, zj = zero, crho = uno;
+                if (mode == 1) {
+                    xj = u00 * std::sin(due * pi * x) * std::cos(due * pi * y) * std::cos(due * pi * z);
+                    yj = -u00 * std::cos(due * pi * x) * std::sin(due * pi * y) * std::cos(due * pi * z);
+                    crho = uno + ((u00 * u00) / (qua * qua * tre)) * (std::cos(qua * pi * x) + std::cos(qua * pi * y)) *
+                                     (std::cos(qua * pi * z) + due);
+                }
+                const real_t cvsq = xj * xj + yj * yj + zj * zj;
+                // c.u per direction in the reference's grouping (SRC/init.F90:159-177)
+                const real_t lin[19] = {xj - yj, xj - zj, xj + yj, xj + zj, xj, zj, yj + zj, yj, yj - zj,
+                                        -xj - yj, -xj - zj, -xj + yj, -xj + zj, -xj, -zj, -yj - zj, -yj, -yj + zj,
+                                        zero};
+                const real_t sq[19] = {xj - yj, xj - zj, xj + yj, xj + zj, xj, zj, yj + zj, yj, yj - zj,
+                                       xj + yj, xj + zj, xj - yj, xj - zj, xj, zj, yj + zj, yj, yj - zj, zero};
+                const size_t id = (size_t)i + (size_t)nx * j + nxy * k;
+                for (int d = 0; d < 19; ++d) {
+                    const real_t c = rf * lin[d] + qf * (tre * sq[d] * sq[d] - cvsq);
+                    h[d][id] = crho * w[d] * (cte1 + c);
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+// ---- device memory plumbing for hosts that have no other HIP runtime ----
+int bgk_dev_malloc(void** ptr, size_t bytes)
+{
+    BGK_HIP_CHECK(hipMalloc(ptr, bytes));
+    return 0;
+}
+int bgk_dev_free(void* ptr)
+{
+    BGK_HIP_CHECK(hipFree(ptr));
+    return 0;
+}
+int bgk_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    BGK_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+int bgk_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    BGK_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+int bgk_device_count(int* n)
+{
+    BGK_HIP_CHECK(hipGetDeviceCount(n));
+    return 0;
+}
+
+}  // extern "C"

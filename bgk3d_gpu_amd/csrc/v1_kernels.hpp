@@ -1,0 +1,282 @@
+// gfx950 kernels behind the reference-compatible C-ABI (v1).
+//
+// Memory is owned by the caller and laid out as in the reference
+// (SRC/alloca.F90:153-191): 19 (plus 18) separate column-major arrays of
+// shape (0:l+1+ipad, 0:m+1+jpad, 0:n+1), element (i,j,k) at i + nx*j + nxy*k.
+//
+// Design (not a translation of the reference's one-thread-per-cell 1-D
+// launch with 64-bit div/mod):
+//  * collide: wave64 rows along i (blockDim 64 x 4), a 3-D grid so no
+//    division is needed for the cell index, all 19 populations pulled in one
+//    pass with coalesced loads, results written with non-temporal stores
+//    (the output lattice is not read again in this step);
+//  * every population element is read exactly once per step (pull streaming
+//    is a permutation), so HBM traffic is 19 loads + 19 stores per cell;
+//  * boundary kernels cover only the surface and keep the reference's face
+//    order, since edge cells rely on the last writer.
+#pragma once
+
+#include "common.hpp"
+#include "d3q19.hpp"
+
+namespace bgk {
+
+template <typename R>
+struct Lattice19 {
+    R* p[19];
+};
+
+struct Geometry {
+    int l, m, n;
+    int nx;      // leading dimension size(a,1)
+    long nxy;    // nx * size(a,2)
+};
+
+__device__ __forceinline__ long gidx(const Geometry& g, int i, int j, int k)
+{
+    return (long)i + (long)g.nx * (long)j + g.nxy * (long)k;
+}
+
+// -------------------------------------------------------------------------
+// Fused pull-stream + BGK collide, one step (SRC/col_MC.F90:126-536).
+// Reads a[0..18], writes b[0..17] and a[18] in place.
+// -------------------------------------------------------------------------
+constexpr int kColBX = 64;
+constexpr int kColBY = 4;
+
+template <typename R, int FMODE>
+__global__ __launch_bounds__(kColBX* kColBY) void col_mc_kernel(Lattice19<R> a, Lattice19<R> b, Geometry g,
+                                                               R omega, R fgrad, R ux, R uy, R uz,
+                                                               CollideConsts<R> cst)
+{
+    const int i = 1 + blockIdx.x * kColBX + threadIdx.x;
+    const int j = 1 + blockIdx.y * kColBY + threadIdx.y;
+    const int k = 1 + blockIdx.z;
+    if (i > g.l || j > g.m) return;
+    const long s = gidx(g, i, j, k);
+    const long nx = g.nx, nxy = g.nxy;
+
+    R x[19];
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        const long off = (long)cx_of(d) + nx * (long)cy_of(d) + nxy * (long)cz_of(d);
+        x[d] = __builtin_nontemporal_load(a.p[d] + (s - off));
+    }
+    R fx, fy, fz;
+    force_terms<R, FMODE>(density_ref(x, cst.cte1), fgrad, ux, uy, uz, fx, fy, fz);
+    R o[19];
+    bgk_collide_ref(x, omega, fx, fy, fz, cst, o);
+#pragma unroll
+    for (int d = 0; d < 18; ++d) __builtin_nontemporal_store(o[d], b.p[d] + s);
+    __builtin_nontemporal_store(o[18], a.p[18] + s);
+}
+
+// -------------------------------------------------------------------------
+// Boundary faces.  A(d) is population d (1-based, as in the Fortran).
+// -------------------------------------------------------------------------
+#define BGK_A(d, i, j, k) a.p[(d)-1][gidx(g, (i), (j), (k))]
+
+// bcond_driven x faces, j=1..m, k=1..n (SRC/bcond.driven.F90:105-119)
+template <typename R>
+__global__ void driven_x(Lattice19<R> a, Geometry g)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.m * g.n) return;
+    const int j = 1 + t % g.m, k = 1 + t / g.m, l = g.l, l1 = g.l + 1;
+    BGK_A(12, l1, j - 1, k) = BGK_A(1, l, j, k);
+    BGK_A(13, l1, j, k - 1) = BGK_A(2, l, j, k);
+    BGK_A(10, l1, j + 1, k) = BGK_A(3, l, j, k);
+    BGK_A(11, l1, j, k + 1) = BGK_A(4, l, j, k);
+    BGK_A(14, l1, j, k) = BGK_A(5, l, j, k);
+    BGK_A(3, 0, j - 1, k) = BGK_A(10, 1, j, k);
+    BGK_A(4, 0, j, k - 1) = BGK_A(11, 1, j, k);
+    BGK_A(1, 0, j + 1, k) = BGK_A(12, 1, j, k);
+    BGK_A(2, 0, j, k + 1) = BGK_A(13, 1, j, k);
+    BGK_A(5, 0, j, k) = BGK_A(14, 1, j, k);
+}
+
+// bcond_driven y faces, i=1..l, k=1..n (SRC/bcond.driven.F90:169-183)
+template <typename R>
+__global__ void driven_y(Lattice19<R> a, Geometry g)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.l * g.n) return;
+    const int i = 1 + t % g.l, k = 1 + t / g.l, m = g.m, m1 = g.m + 1;
+    BGK_A(8, i, 0, k) = BGK_A(17, i, 1, k);
+    BGK_A(12, i + 1, 0, k) = BGK_A(1, i, 1, k);
+    BGK_A(3, i - 1, 0, k) = BGK_A(10, i, 1, k);
+    BGK_A(7, i, 0, k - 1) = BGK_A(16, i, 1, k);
+    BGK_A(9, i, 0, k + 1) = BGK_A(18, i, 1, k);
+    BGK_A(10, i + 1, m1, k) = BGK_A(3, i, m, k);
+    BGK_A(16, i, m1, k + 1) = BGK_A(7, i, m, k);
+    BGK_A(17, i, m1, k) = BGK_A(8, i, m, k);
+    BGK_A(18, i, m1, k - 1) = BGK_A(9, i, m, k);
+    BGK_A(1, i - 1, m1, k) = BGK_A(12, i, m, k);
+}
+
+// bcond_driven z faces + lid, i=1..l, j=1..m (SRC/bcond.driven.F90:233-247);
+// `lid` = 1 moves the top wall (driven cavity), 0 makes it a plain no-slip
+// wall (the channel walls of SRC/bcond.channel.F90:244-267).
+template <typename R>
+__global__ void walls_z(Lattice19<R> a, Geometry g, R force, int lid)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.l * g.m) return;
+    const int i = 1 + t % g.l, j = 1 + t / g.l, n = g.n, n1 = g.n + 1;
+    BGK_A(6, i, j, 0) = BGK_A(15, i, j, 1);
+    BGK_A(4, i - 1, j, 0) = BGK_A(11, i, j, 1);
+    BGK_A(7, i, j - 1, 0) = BGK_A(16, i, j, 1);
+    BGK_A(13, i + 1, j, 0) = BGK_A(2, i, j, 1);
+    BGK_A(18, i, j + 1, 0) = BGK_A(9, i, j, 1);
+    BGK_A(15, i, j, n1) = BGK_A(6, i, j, n);
+    if (lid) {
+        BGK_A(2, i - 1, j, n1) = BGK_A(13, i, j, n) + force;
+    } else {
+        BGK_A(2, i - 1, j, n1) = BGK_A(13, i, j, n);
+    }
+    BGK_A(9, i, j - 1, n1) = BGK_A(18, i, j, n);
+    if (lid) {
+        BGK_A(11, i + 1, j, n1) = BGK_A(4, i, j, n) - force;
+    } else {
+        BGK_A(11, i + 1, j, n1) = BGK_A(4, i, j, n);
+    }
+    BGK_A(16, i, j + 1, n1) = BGK_A(7, i, j, n);
+}
+
+// periodic x face over j=0..m+1, k=0..n+1 (SRC/bcond.periodic.F90:105-119)
+template <typename R>
+__global__ void periodic_x(Lattice19<R> a, Geometry g)
+{
+    const int nj = g.m + 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nj * (g.n + 2)) return;
+    const int j = t % nj, k = t / nj, l = g.l, l1 = g.l + 1;
+#pragma unroll
+    for (int d = 1; d <= 5; ++d) BGK_A(d, 0, j, k) = BGK_A(d, l, j, k);
+#pragma unroll
+    for (int d = 10; d <= 14; ++d) BGK_A(d, l1, j, k) = BGK_A(d, 1, j, k);
+}
+
+// periodic y face over i=0..l+1, k=0..n+1 (SRC/bcond.periodic.F90:171-185)
+template <typename R>
+__global__ void periodic_y(Lattice19<R> a, Geometry g)
+{
+    const int ni = g.l + 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * (g.n + 2)) return;
+    const int i = t % ni, k = t / ni, m = g.m, m1 = g.m + 1;
+    BGK_A(3, i, 0, k) = BGK_A(3, i, m, k);
+    BGK_A(7, i, 0, k) = BGK_A(7, i, m, k);
+    BGK_A(8, i, 0, k) = BGK_A(8, i, m, k);
+    BGK_A(9, i, 0, k) = BGK_A(9, i, m, k);
+    BGK_A(12, i, 0, k) = BGK_A(12, i, m, k);
+    BGK_A(1, i, m1, k) = BGK_A(1, i, 1, k);
+    BGK_A(10, i, m1, k) = BGK_A(10, i, 1, k);
+    BGK_A(16, i, m1, k) = BGK_A(16, i, 1, k);
+    BGK_A(17, i, m1, k) = BGK_A(17, i, 1, k);
+    BGK_A(18, i, m1, k) = BGK_A(18, i, 1, k);
+}
+
+// periodic z face over i=0..l+1, j=0..m+1 (SRC/bcond.periodic.F90:238-252)
+template <typename R>
+__global__ void periodic_z(Lattice19<R> a, Geometry g)
+{
+    const int ni = g.l + 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * (g.m + 2)) return;
+    const int i = t % ni, j = t / ni, n = g.n, n1 = g.n + 1;
+    BGK_A(4, i, j, 0) = BGK_A(4, i, j, n);
+    BGK_A(6, i, j, 0) = BGK_A(6, i, j, n);
+    BGK_A(7, i, j, 0) = BGK_A(7, i, j, n);
+    BGK_A(13, i, j, 0) = BGK_A(13, i, j, n);
+    BGK_A(18, i, j, 0) = BGK_A(18, i, j, n);
+    BGK_A(2, i, j, n1) = BGK_A(2, i, j, 1);
+    BGK_A(9, i, j, n1) = BGK_A(9, i, j, 1);
+    BGK_A(11, i, j, n1) = BGK_A(11, i, j, 1);
+    BGK_A(15, i, j, n1) = BGK_A(15, i, j, 1);
+    BGK_A(16, i, j, n1) = BGK_A(16, i, j, 1);
+}
+
+// inflow/outflow x faces over j=0..m+1, k=0..n+1
+// (SRC/bcond.inflow.F90:131-181).  Note the reference writes crho*p2*(...)
+// = (crho*p2)*(...) and forms c.u terms as rf*(-xj-yj) etc.; each line below
+// keeps that grouping.
+template <typename R>
+__global__ void inflow_x(Lattice19<R> a, Geometry g, R u_inflow, R rf, R qf, R p1, R p2, R cte1)
+{
+    const int nj = g.m + 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nj * (g.n + 2)) return;
+    const int j = t % nj, k = t / nj, l = g.l, l1 = g.l + 1;
+    // `3.0` is a literal in the reference expression, not the tre argument
+    const R three = R(3), crho = R(1), rhoinv = R(1);
+    R f[19];
+#pragma unroll
+    for (int d = 0; d < 19; ++d) f[d] = a.p[d][gidx(g, l, j, k)];
+    R xj = +(f[0] + f[1] + f[2] + f[3] + f[4] - f[9] - f[10] - f[11] - f[12] - f[13]) * rhoinv;
+    R yj = +(f[2] + f[6] + f[7] + f[8] + f[11] - f[0] - f[9] - f[15] - f[16] - f[17]) * rhoinv;
+    R zj = +(f[3] + f[5] + f[6] + f[12] + f[17] - f[1] - f[8] - f[10] - f[14] - f[15]) * rhoinv;
+    R cvsq = xj * xj + yj * yj + zj * zj;
+    const R cx10 = rf * (-xj - yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
+    const R cx11 = rf * (-xj - zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
+    const R cx12 = rf * (-xj + yj) + qf * (three * (xj - yj) * (xj - yj) - cvsq);
+    const R cx13 = rf * (-xj + zj) + qf * (three * (xj - zj) * (xj - zj) - cvsq);
+    const R cx14 = rf * (-xj) + qf * (three * (xj) * (xj)-cvsq);
+    BGK_A(10, l1, j, k) = crho * p2 * (cte1 + cx10);
+    BGK_A(11, l1, j, k) = crho * p2 * (cte1 + cx11);
+    BGK_A(12, l1, j, k) = crho * p2 * (cte1 + cx12);
+    BGK_A(13, l1, j, k) = crho * p2 * (cte1 + cx13);
+    BGK_A(14, l1, j, k) = crho * p1 * (cte1 + cx14);
+    xj = u_inflow;
+    yj = R(0);
+    zj = R(0);
+    cvsq = xj * xj + yj * yj + zj * zj;
+    const R cx01 = rf * (xj - yj) + qf * (three * (xj - yj) * (xj - yj) - cvsq);
+    const R cx02 = rf * (xj - zj) + qf * (three * (xj - zj) * (xj - zj) - cvsq);
+    const R cx03 = rf * (xj + yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
+    const R cx04 = rf * (xj + zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
+    const R cx05 = rf * (xj) + qf * (three * (xj) * (xj)-cvsq);
+    BGK_A(1, 0, j, k) = crho * p2 * (cte1 + cx01);
+    BGK_A(2, 0, j, k) = crho * p2 * (cte1 + cx02);
+    BGK_A(3, 0, j, k) = crho * p2 * (cte1 + cx03);
+    BGK_A(4, 0, j, k) = crho * p2 * (cte1 + cx04);
+    BGK_A(5, 0, j, k) = crho * p1 * (cte1 + cx05);
+}
+
+// Obstacle bounce-back (SRC/bcond.obs.F90:101-122): for every solid cell S
+// of the box, a_d(S) = a_opp(d)(S + c_d), which is what the fluid cell
+// X = S + c_d pulls as population d next (half-way bounce-back of its own
+// opposite population).  Only links whose neighbour S + c_d is fluid are
+// written, so every read comes from a fluid cell this kernel never writes:
+// the result is deterministic and fluid cells see exactly the reference's
+// values.  (The reference's own
+// GPU kernel and CPU loop also read solid neighbours they overwrite in the
+// same pass; those values only ever reach other solid cells.)
+template <typename R>
+__global__ void obstacle_bb(Lattice19<R> a, Geometry g, const int* __restrict__ obs, int imin, int imax, int jmin,
+                            int jmax, int kmin, int kmax)
+{
+    const int ni = imax - imin + 1, nj = jmax - jmin + 1, nk = kmax - kmin + 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ni * nj * nk) return;
+    const int i = imin + (int)(t % ni);
+    const int j = jmin + (int)((t / ni) % nj);
+    const int k = kmin + (int)(t / ((long)ni * nj));
+    auto solid = [&](int ii, int jj, int kk) -> bool {
+        if (ii < 1 || ii > g.l || jj < 1 || jj > g.m || kk < 1 || kk > g.n) return false;
+        return obs[(long)(ii - 1) + (long)g.l * (jj - 1) + (long)g.l * g.m * (kk - 1)] == 1;
+    };
+    if (!solid(i, j, k)) return;
+    const long c = gidx(g, i, j, k);
+#pragma unroll
+    for (int d = 0; d < 18; ++d) {
+        // reference: a_d(i,j,k) = a_opp(d)(i + c_d)
+        const int ii = i + cx_of(d), jj = j + cy_of(d), kk = k + cz_of(d);
+        if (solid(ii, jj, kk)) continue;
+        a.p[d][c] = a.p[opp_of(d)][gidx(g, ii, jj, kk)];
+    }
+}
+
+#undef BGK_A
+
+}  // namespace bgk

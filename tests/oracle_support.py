@@ -1,0 +1,248 @@
+"""Test-side access to the CPU oracle and the compiled reference harness.
+
+The oracle (oracle/_build/libbgk_oracle.so) is a C restatement of the
+reference's Fortran CPU path; the reference harness binaries
+(oracle/_ref/<case>_<prec>/ref_harness.x) are the reference's own routines
+compiled from its sources.  Both are checkers: nothing under
+``bgk3d_gpu_amd`` imports this module.
+
+Arrays follow the reference layout: population d (0-based) of a lattice
+is ``A[d]`` with numpy shape (nz, ny, nx) -- the Fortran column-major array
+(0:nx-1, 0:ny-1, 0:nz-1) seen in C order (SRC/alloca.F90:153-191).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import shutil
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parents[1]
+ORACLE_SO = REPO / "oracle" / "_build" / "libbgk_oracle.so"
+REF_DIR = REPO / "oracle" / "_ref"
+
+# u_inflow is hard-wired as the default-real literal 0.1 in the reference
+# (SRC/bcond.inflow.F90:81), so the DP build runs with (double)0.1f.
+U_INFLOW_REF = float(np.float32(0.1))
+
+# boundary family codes shared with oracle_run_*: driven, periodic, channel,
+# inflow (+ obstacle)
+BC_CODE = {"ldc": 0, "periodic": 1, "channel": 2, "inflow": 3}
+
+_lib = None
+
+
+def oracle_lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            subprocess.run(["make", "-C", str(REPO / "oracle"), "oracle"], check=True,
+                           stdout=subprocess.DEVNULL)
+        _lib = ctypes.CDLL(str(ORACLE_SO))
+    return _lib
+
+
+def np_dtype(prec: str):
+    return np.float32 if prec == "sp" else np.float64
+
+
+def ctype(prec: str):
+    return ctypes.c_float if prec == "sp" else ctypes.c_double
+
+
+def sfx(prec: str) -> str:
+    return "f32" if prec == "sp" else "f64"
+
+
+def new_lattice(l, m, n, prec="sp", ipad=0, jpad=0, poison=True):
+    nx, ny, nz = l + 2 + ipad, m + 2 + jpad, n + 2
+    dt = np_dtype(prec)
+    fill = np.finfo(dt).max if poison else 0
+    return np.full((19, nz, ny, nx), fill, dtype=dt)
+
+
+def ptrs(A):
+    """Host array of the 19 base pointers (the Fortran pa(1:19))."""
+    P = (ctypes.c_void_p * 19)()
+    for d in range(19):
+        P[d] = A[d].ctypes.data
+    return P
+
+
+def fn(name, prec):
+    return getattr(oracle_lib(), f"oracle_{name}_{sfx(prec)}")
+
+
+def hencol(svisc, u0, m, prec="sp"):
+    ct = ctype(prec)
+    om, fg = ct(), ct()
+    f = fn("hencol", prec)
+    f.argtypes = [ct, ct, ctypes.c_int, ctypes.POINTER(ct), ctypes.POINTER(ct)]
+    f(ct(svisc), ct(u0), m, ctypes.byref(om), ctypes.byref(fg))
+    return om.value, fg.value
+
+
+def init(A, l, m, n, prec="sp", mode=0, u00=0.0):
+    nx, ny = A.shape[3], A.shape[2]
+    ct = ctype(prec)
+    f = fn("init", prec)
+    f.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ct]
+    f(ptrs(A), l, m, n, nx, ny, mode, ct(u00))
+
+
+def build_obs(l, m, n):
+    obs = np.zeros((n, m, l), dtype=np.int32)
+    box = (ctypes.c_int * 6)()
+    rad = ctypes.c_int()
+    f = oracle_lib().oracle_build_obs
+    f.restype = ctypes.c_long
+    f(obs.ctypes.data_as(ctypes.c_void_p), l, m, n, box, ctypes.byref(rad))
+    return obs, tuple(box), rad.value
+
+
+def col_mc(A, B, l, m, n, omega, prec="sp", force_mode=0, fgrad=0.0, fxyz=(0.0, 0.0, 0.0), nthreads=1):
+    nx, ny = A.shape[3], A.shape[2]
+    ct = ctype(prec)
+    f = fn("col_mc", prec)
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5 + [ct, ctypes.c_int, ct, ct, ct, ct,
+                                                                           ctypes.c_int]
+    f(ptrs(A), ptrs(B), l, m, n, nx, ny, ct(omega), force_mode, ct(fgrad), ct(fxyz[0]), ct(fxyz[1]),
+      ct(fxyz[2]), nthreads)
+
+
+def swap(A, B):
+    """Fortran pointer swap of populations 1..18 (SRC/col_MC.F90:539-596)."""
+    tmp = A[:18].copy()
+    A[:18] = B[:18]
+    B[:18] = tmp
+
+
+def bcond(case, A, l, m, n, prec="sp", u00=0.0, u_inflow=U_INFLOW_REF, obs=None, box=None):
+    nx, ny = A.shape[3], A.shape[2]
+    ct = ctype(prec)
+    P = ptrs(A)
+    if case == "ldc":
+        f = fn("bcond_driven", prec)
+        f.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [ct]
+        f(P, l, m, n, nx, ny, ct(u00))
+    elif case == "periodic":
+        fn("bcond_periodic", prec)(P, l, m, n, nx, ny)
+    elif case == "channel":
+        fn("bcond_channel", prec)(P, l, m, n, nx, ny)
+    elif case == "inflow":
+        f = fn("bcond_inflow", prec)
+        f.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [ct]
+        f(P, l, m, n, nx, ny, ct(u_inflow))
+    if obs is not None:
+        f = fn("bcond_obs", prec)
+        f(P, obs.ctypes.data_as(ctypes.c_void_p), l, m, n, nx, ny, *box)
+
+
+def run(case, A, B, l, m, n, steps, prec="sp", svisc=0.064, u0=0.1, obs=None, box=None, u_inflow=U_INFLOW_REF,
+        nthreads=1):
+    """Reference time loop (SRC/bgk3d.F90:169-196) on host arrays A, B.
+
+    Pointer swaps are done on the pointer tables; on return the current
+    lattice is copied back into A (populations 1..18 may live in B)."""
+    nx, ny = A.shape[3], A.shape[2]
+    ct = ctype(prec)
+    omega, fgrad = hencol(svisc, u0, m, prec)
+    # build_bcond (SRC/build.bcond.F90:29-35): lid velocity vs volume force
+    if case == "channel":
+        u00, force_mode = 0.0, 1
+    else:
+        u00, force_mode, fgrad = u0, 0, 0.0
+    PA, PB = ptrs(A), ptrs(B)
+    f = fn("run", prec)
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6 + [ct, ct, ctypes.c_int, ct, ct,
+                                                                           ctypes.c_void_p, ctypes.c_void_p,
+                                                                           ctypes.c_int, ctypes.c_int]
+    obs_p = obs.ctypes.data_as(ctypes.c_void_p) if obs is not None else None
+    box_p = (ctypes.c_int * 6)(*box) if box is not None else None
+    f(PA, PB, l, m, n, nx, ny, BC_CODE[case], ct(omega), ct(u00), force_mode, ct(fgrad), ct(u_inflow), obs_p,
+      box_p, steps, nthreads)
+    # gather the current lattice back into A
+    cur = [None] * 19
+    addr = {A[d].ctypes.data: ("A", d) for d in range(19)}
+    addr.update({B[d].ctypes.data: ("B", d) for d in range(19)})
+    for d in range(19):
+        src, sd = addr[PA[d]]
+        cur[d] = (A if src == "A" else B)[sd].copy()
+    for d in range(19):
+        A[d] = cur[d]
+
+
+def diagno(A, l, m, n, prec="sp"):
+    nx, ny = A.shape[3], A.shape[2]
+    out = (ctypes.c_double * 5)()
+    fn("diagno", prec)(ptrs(A), l, m, n, nx, ny, out)
+    return np.array(out[:])
+
+
+def varm(A, l, m, n, prec="sp"):
+    nx, ny = A.shape[3], A.shape[2]
+    dt = np_dtype(prec)
+    u, w, v = (np.zeros(n, dt) for _ in range(3))
+    den = np.zeros(m, dt)
+    fn("varm", prec)(ptrs(A), l, m, n, nx, ny, *(x.ctypes.data_as(ctypes.c_void_p) for x in (u, w, v, den)))
+    return u, w, v, den
+
+
+def prof_k(A, l, m, n, ic, jc, prec="sp"):
+    nx, ny = A.shape[3], A.shape[2]
+    out = np.zeros((n, 4), np_dtype(prec))
+    fn("prof_k", prec)(ptrs(A), l, m, n, nx, ny, ic, jc, out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+def draglift(A, obs, l, m, n, radius, u_inflow=U_INFLOW_REF, prec="sp"):
+    nx, ny = A.shape[3], A.shape[2]
+    ct = ctype(prec)
+    out = np.zeros(3, np_dtype(prec))
+    f = fn("draglift", prec)
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6 + [ct, ctypes.c_void_p]
+    f(ptrs(A), obs.ctypes.data_as(ctypes.c_void_p), l, m, n, nx, ny, radius, ct(u_inflow),
+      out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# compiled reference harness (only where oracle/_ref was built)
+# ---------------------------------------------------------------------------
+
+def ref_available(case="ldc", prec="sp", omp=False) -> bool:
+    return (REF_DIR / f"{case}_{prec}{'_omp' if omp else ''}" / "ref_harness.x").exists()
+
+
+def run_ref(case, l, m, n, steps, prec="sp", svisc=0.064, u0=0.1, icheck=None, init_v=0, ipad=0, jpad=0,
+            mode=0, A_in=None, omp=False, threads=None, keep_dir=None):
+    """Run the reference harness; returns (A_out, outdir_texts, stdout)."""
+    exe = REF_DIR / f"{case}_{prec}{'_omp' if omp else ''}" / "ref_harness.x"
+    d = Path(keep_dir) if keep_dir else Path(tempfile.mkdtemp(prefix="bgkref_"))
+    d.mkdir(parents=True, exist_ok=True)
+    try:
+        if A_in is not None:
+            init_v = -1
+            np.ascontiguousarray(A_in).tofile(d / "in.bin")
+        env = dict(os.environ)
+        if threads:
+            env["OMP_NUM_THREADS"] = str(threads)
+        cmd = [str(exe), str(d), str(l), str(m), str(n), repr(float(svisc)), repr(float(u0)), str(steps),
+               str(icheck or max(steps, 1)), str(init_v), str(ipad), str(jpad), str(mode)]
+        out = subprocess.run(cmd, check=True, capture_output=True, text=True, env=env).stdout
+        nx, ny, nz = l + 2 + ipad, m + 2 + jpad, n + 2
+        A = np.fromfile(d / "out.bin", dtype=np_dtype(prec)).reshape(19, nz, ny, nx)
+        texts = {}
+        for name in ("diagno.dat", "prof_k.dat", "u_med.dat", "drag.lift.dat"):
+            p = d / name
+            texts[name] = p.read_text() if p.exists() else ""
+        if (d / "obs.bin").exists():
+            texts["obs"] = np.fromfile(d / "obs.bin", dtype=np.int32).reshape(n, m, l)
+        return A, texts, out
+    finally:
+        if keep_dir is None:
+            shutil.rmtree(d, ignore_errors=True)
