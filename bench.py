@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Benchmark of the D3Q19 BGK time step on one MI355X.
+
+Workload (the bandwidth-roofline case): 3-D Taylor-Green vortex, periodic
+256^3, FP32 storage and arithmetic, nu = 0.01 (tau = 0.53), U0 = 0.05.  A
+"step" is one full time step: boundaries (periodic halo) + fused pull-stream
++ BGK collide, exactly the reference's loop body (SRC/bgk3d.F90:169-196).
+
+value  = whole-job MLUPS = l*m*n*steps / time / 1e6 (SRC/finalize.F90:106-107)
+roofline: dominant kernel (the fused collide), algorithmic bytes per launch =
+          38 * 4 B * l*m*n (19 loads + 19 stores per interior cell), divided
+          by the kernel's average duration measured with hipEvents around each
+          launch in the timed region, against the 8 TB/s HBM peak.
+cpu_baseline: the reference's own Fortran CPU routines (oracle/_ref,
+          serial build) on a bounded sample of the same workload.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+METRIC = "MLUPS + % HBM roofline (FP32 vs FP16-store); sweep-server p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def stream_bandwidth(lib, rt, nbytes=4 << 30, reps=10):
+    """Measured copy bandwidth (read + write bytes / time) of a plain copy kernel."""
+    n = nbytes // 4
+    src, dst = rt.DeviceBuffer(nbytes, lib), rt.DeviceBuffer(nbytes, lib)
+    try:
+        lib.bgk_stream_copy(src.ptr, dst.ptr, n)
+        e0, e1 = rt.Event(lib), rt.Event(lib)
+        e0.record()
+        for _ in range(reps):
+            lib.bgk_stream_copy(src.ptr, dst.ptr, n)
+        e1.record()
+        ms = e0.elapsed_ms(e1)
+        return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
+    finally:
+        src.close()
+        dst.close()
+
+
+def cpu_baseline(case, steps):
+    """Reference serial build timed on the host (bounded sample)."""
+    exe = REPO / "oracle" / "_ref" / "periodic_sp" / "ref_harness.x"
+    if exe.exists():
+        with tempfile.TemporaryDirectory() as d:
+            t0 = time.time()
+            out = subprocess.run([str(exe), d, str(case.l), str(case.m), str(case.n), repr(case.svisc),
+                                  repr(case.u0), str(steps), str(steps + 1), "1", "0", "0", "3"],
+                                 capture_output=True, text=True, timeout=600, env=dict(os.environ,
+                                                                                         OMP_NUM_THREADS="1"))
+            mlups = None
+            for line in out.stdout.splitlines():
+                if line.startswith("# Mlups"):
+                    mlups = float(line.split()[-1])
+            if mlups is None:
+                raise RuntimeError(f"reference harness failed: {out.stdout[-500:]} {out.stderr[-500:]}")
+            return {"value": mlups, "unit": "MLUPS", "cores": 1, "kind": "reference",
+                    "sample": f"reference Fortran CPU routines (amdflang -O3, serial), Taylor-Green "
+                              f"{case.l}x{case.m}x{case.n} FP32, {steps} steps; wall {time.time() - t0:.1f}s incl. init"}
+    # fall back to the C oracle with host threads
+    sys.path.insert(0, str(REPO / "tests"))
+    import oracle_support as O
+    threads = min(16, os.cpu_count() or 1)
+    A = O.new_lattice(case.l, case.m, case.n, "sp")
+    B = O.new_lattice(case.l, case.m, case.n, "sp")
+    O.init(A, case.l, case.m, case.n, "sp", mode=1, u00=case.u0)
+    t0 = time.perf_counter()
+    O.run("periodic", A, B, case.l, case.m, case.n, steps, prec="sp", svisc=case.svisc, u0=case.u0,
+          nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": case.cells * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"C oracle, OpenMP {threads} threads, Taylor-Green {case.l}^3 FP32, {steps} steps"}
+
+
+def load_traffic(workload):
+    p = REPO / "profiles" / "traffic.json"
+    if p.exists():
+        data = json.loads(p.read_text())
+        return data.get(workload)
+    return None
+
+
+def bench_v1(case, prec, steps, warmup):
+    """Reference-compatible ABI driven the reference's way (one
+    bcond_periodic_gpu + col_mc_gpu per step, caller-owned arrays)."""
+    from bgk3d_gpu_amd import runtime as rt
+    from bgk3d_gpu_amd.simulation import Simulation
+
+    sim = Simulation(case, prec, poison=False)
+    lib = sim.lib
+    try:
+        sim.step(warmup)
+        sim.sync()
+        ev_start, ev_stop = rt.Event(lib), rt.Event(lib)
+        ev_start.record()
+        sim.step(steps)
+        ev_stop.record()
+        total_ms = ev_start.elapsed_ms(ev_stop)
+    finally:
+        sim.close()
+    return total_ms
+
+
+def bench_engine(case, storage, steps, warmup):
+    from bgk3d_gpu_amd.engine import Engine
+
+    with Engine(case, storage) as eng:
+        eng.step(warmup)
+        eng.sync()
+        total_ms, kern_ms = eng.time_steps(steps)
+        diag = eng.diagno()
+        info = eng.info()
+    return total_ms, kern_ms, diag, info
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--prec", default="sp", choices=["sp", "dp"])
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP64/FP16/BF16/v1 side measurements")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from bgk3d_gpu_amd import runtime as rt
+    from bgk3d_gpu_amd.cases import Case
+
+    n = args.size
+    case = Case("periodic", n, n, n, svisc=0.01, u0=0.05, init=1)
+    storage = "f32" if args.prec == "sp" else "f64"
+    workload = f"taylor_green_{n}^3_{'fp32' if args.prec == 'sp' else 'fp64'}"
+    log(f"workload {workload} steps={args.steps} warmup={args.warmup}")
+    total_ms, kern_avg_ms, diag, info = bench_engine(case, storage, args.steps, args.warmup)
+    if not np.all(np.isfinite(diag)):
+        raise SystemExit(f"non-finite diagnostics {diag}")
+    s_bytes = 4 if args.prec == "sp" else 8
+    ms_per_step = total_ms / args.steps
+    mlups = case.cells * args.steps / (total_ms * 1e-3) / 1e6
+    alg_bytes = 38 * s_bytes * case.cells
+    achieved = alg_bytes / (kern_avg_ms * 1e-3) / 1e9
+    log(f"MLUPS {mlups:.0f}; collide avg {kern_avg_ms * 1e3:.1f} us -> {achieved:.0f} GB/s")
+    forms = {}
+    if not args.no_forms:
+        fsteps = max(10, args.steps // 4)
+        for st, sb in (("f64", 8), ("f16", 2), ("bf16", 2)):
+            if st == storage:
+                continue
+            t, k, d, _ = bench_engine(case, st, fsteps, 5)
+            forms[st] = {"mlups": round(case.cells * fsteps / (t * 1e-3) / 1e6, 1),
+                         "collide_us": round(k * 1e3, 2),
+                         "alg_GBs": round(38 * sb * case.cells / (k * 1e-3) / 1e9, 1)}
+            log(f"{st}: {forms[st]}")
+        t = bench_v1(case, args.prec, fsteps, 5)
+        forms["v1_abi_" + storage] = {"mlups": round(case.cells * fsteps / (t * 1e-3) / 1e6, 1),
+                                      "note": "reference layout (nx=l+2), halo kernels per step"}
+        log(f"v1: {forms['v1_abi_' + storage]}")
+        if "f16" in forms:
+            forms["f16_over_" + storage] = round(forms["f16"]["mlups"] / mlups, 3)
+    stream = None
+    if not args.no_stream:
+        stream = stream_bandwidth(rt.v1("sp"), rt)
+        log(f"measured copy bandwidth {stream:.0f} GB/s")
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(case, args.cpu_steps)
+        log(f"cpu baseline {cpu['value']:.2f} MLUPS ({cpu['kind']}, {cpu['cores']} cores)")
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.prec == "sp" else "f64",
+        "data": "synthetic (analytic Taylor-Green initial state, no dataset)",
+        "config": {"workload": workload, "grid": [n, n, n], "boundary": "periodic", "tau": 0.53, "U0": 0.05,
+                   "path": "handle API: aligned layout, periodic boundary folded into the fused step",
+                   "layout_nx": info["nx"], "device_bytes": info["bytes"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(workload),
+                     "kernel": "collide_fused", "kernel_avg_us": round(kern_avg_ms * 1e3, 2),
+                     "alg_bytes_per_launch": alg_bytes,
+                     "measured_copy_GBs": round(stream, 1) if stream else None,
+                     "frac_of_measured_copy": round(achieved / stream, 4) if stream else None},
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,168 @@
+// Fused pull-stream + BGK collide with the boundary conditions folded into
+// the pull -- the one hot kernel behind both C-ABIs.
+//
+// One step of SRC/col_MC.F90:126-536 for every interior cell.  Population d
+// of cell x is pulled from x - c_d.  When that source lies outside the
+// interior, the value the reference's boundary routine would have left in
+// the halo is produced directly:
+//   AX_HALO      read the halo as it is (the caller filled it: reference
+//                semantics of bcond_*_gpu, and the engine's inflow/outflow
+//                faces);
+//   AX_PERIODIC  wrap the coordinate (SRC/bcond.periodic.F90:105-252);
+//   AX_WALL      half-way bounce-back: the cell's own opposite population
+//                (SRC/bcond.driven.F90:105-247, SRC/bcond.channel.F90:255-267);
+//   AX_LID       as AX_WALL, and a pull across the top wall z = n+1 adds
+//                +force to direction 2 and -force to direction 11
+//                (SRC/bcond.driven.F90:243-247; the z face is written last
+//                there, so its value wins on shared edges).
+// A wall on any crossed axis takes precedence over wrapping, as in the
+// reference, where the wall faces are written after the periodic ones.
+// Obstacles: bit d of the cell's link mask set means x - c_d is solid and
+// the pull returns the own opposite population -- the value the reference's
+// bcond_obs stores in the solid cell (SRC/bcond.obs.F90:103-121).
+//
+// Layout: population d is an array (0:nx-1, 0:ny-1, 0:nz-1), element
+// (i,j,k) at i + nx*j + nxy*k.  A wave of 64 lanes covers 64 consecutive
+// cells of one row (j,k).  If `align` is non-zero the row's first wave starts
+// on the `align`-element boundary at or before i = 1 (computed from the
+// output address), so that every wave's loads and stores of the same row
+// hit whole 128-byte lines whatever layout the caller chose; the engine's own
+// layout puts i = 1 on that boundary, making the shift zero.  Loads are plain
+// (neighbouring waves share the lines of the +-1 shifted pulls through L2);
+// stores are non-temporal (the output is not read again this step).
+#pragma once
+
+#include "common.hpp"
+#include "d3q19.hpp"
+#include "storage.hpp"
+
+namespace bgk {
+
+enum AxisBC : int { AX_HALO = 0, AX_PERIODIC = 1, AX_WALL = 2, AX_LID = 3 };
+
+template <typename S>
+struct FusedArgs {
+    typedef typename Storage<S>::compute C;
+    S* a[19];  // current lattice (read), a[18] updated in place
+    S* b[18];  // next lattice, directions 1..18
+    int l, m, n, nx;
+    long nxy;
+    C omega, fgrad, fx, fy, fz, lidforce;
+    CollideConsts<C> cst;
+    // obstacle link masks over the box [oi0, oi0+oni) x ... (bit d: x - c_d solid)
+    const uint32_t* obits;
+    int oi0, oj0, ok0, oni, onj, onk;
+    int align;       // 0: layout already aligned; else elements per 128-byte line
+    uint32_t seed;   // stochastic rounding stream (step-dependent)
+};
+
+constexpr int kWaveRows = 4;  // rows (waves) per 256-thread block
+
+template <int AX>
+__device__ __forceinline__ void axis_fix(int cd, int s, int lo_hi_n, int& shift, bool& bounce, bool& top)
+{
+    // s = coordinate of the source along this axis, valid range 1..lo_hi_n
+    (void)cd;
+    if (AX == AX_HALO) return;
+    const bool below = s < 1, above = s > lo_hi_n;
+    if (!(below || above)) return;
+    if (AX == AX_PERIODIC) {
+        shift = below ? +1 : -1;
+    } else {
+        bounce = true;
+        if (AX == AX_LID && above) top = true;
+    }
+}
+
+// EDGE = the row (j,k) touches a y or z boundary; otherwise only the x
+// direction can leave the interior and the j/k checks are compiled out.
+template <typename S, int BX, int BY, int BZ, bool OBST, bool EDGE>
+__device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, int k, long s,
+                                         typename Storage<S>::compute (&x)[19])
+{
+    typedef typename Storage<S>::compute C;
+    uint32_t ob = 0;
+    if (OBST) {
+        const unsigned ri = (unsigned)(i - p.oi0), rj = (unsigned)(j - p.oj0), rk = (unsigned)(k - p.ok0);
+        if (ri < (unsigned)p.oni && rj < (unsigned)p.onj && rk < (unsigned)p.onk)
+            ob = p.obits[(long)ri + (long)p.oni * ((long)rj + (long)p.onj * rk)];
+    }
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        const int cx = cx_of(d), cy = cy_of(d), cz = cz_of(d);
+        long src = s - ((long)cx + (long)p.nx * cy + p.nxy * cz);
+        bool bounce = false, top = false;
+        if (cx != 0) {
+            int sh = 0;
+            axis_fix<BX>(cx, i - cx, p.l, sh, bounce, top);
+            if (BX == AX_PERIODIC) src += (long)sh * p.l;
+        }
+        if (EDGE && cy != 0) {
+            int sh = 0;
+            axis_fix<BY>(cy, j - cy, p.m, sh, bounce, top);
+            if (BY == AX_PERIODIC) src += (long)sh * p.m * p.nx;
+        }
+        if (EDGE && cz != 0) {
+            int sh = 0;
+            axis_fix<BZ>(cz, k - cz, p.n, sh, bounce, top);
+            if (BZ == AX_PERIODIC) src += (long)sh * p.n * p.nxy;
+        }
+        if (OBST && d < 18) bounce = bounce || ((ob >> d) & 1u);
+        const S* ptr = bounce ? (p.a[opp_of(d)] + s) : (p.a[d] + src);
+        C v = Storage<S>::load(ptr);
+        if (BZ == AX_LID && EDGE) {
+            if (d == 1 && top) v = v + p.lidforce;
+            if (d == 10 && top) v = v - p.lidforce;
+        }
+        x[d] = v;
+    }
+}
+
+template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
+__global__ __launch_bounds__(64 * kWaveRows) void collide_fused(FusedArgs<S> p)
+{
+    typedef typename Storage<S>::compute C;
+    const int j = 1 + blockIdx.y * kWaveRows + threadIdx.y;
+    const int k = 1 + blockIdx.z;
+    if (j > p.m) return;
+    int shift = 0;
+    if (p.align) {
+        const long first = (long)(reinterpret_cast<uintptr_t>(p.b[0]) / sizeof(S)) + 1 + (long)p.nx * j +
+                           p.nxy * (long)k;
+        shift = (int)(first & (long)(p.align - 1));
+    }
+    const int i = 1 - shift + (int)blockIdx.x * 64 + (int)threadIdx.x;
+    if (i < 1 || i > p.l) return;
+    const long s = (long)i + (long)p.nx * j + p.nxy * (long)k;
+
+    C x[19];
+    const bool edge = (BY != AX_HALO && (j == 1 || j == p.m)) || (BZ != AX_HALO && (k == 1 || k == p.n));
+    if (edge)
+        pull_all<S, BX, BY, BZ, OBST, true>(p, i, j, k, s, x);
+    else
+        pull_all<S, BX, BY, BZ, OBST, false>(p, i, j, k, s, x);
+
+    C fx, fy, fz;
+    force_terms<C, FMODE>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
+    C o[19];
+    bgk_collide_ref(x, p.omega, fx, fy, fz, p.cst, o);
+
+    uint32_t r = 0;
+    if (SR) r = hash32(p.seed ^ hash32((uint32_t)s));
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        S* dst = (d < 18 ? p.b[d] : p.a[18]) + s;
+        if (SR) r = hash32(r + 0x9e3779b9U);
+        store_nt(dst, Rounder<S, SR>::cvt(o[d], r));
+    }
+}
+
+// Launch geometry shared by both ABIs.
+inline dim3 fused_grid(int l, int m, int n, int align)
+{
+    const int span = align ? l + align - 1 : l;  // worst-case shifted row span
+    return dim3((unsigned)ceil_div(span, 64), (unsigned)ceil_div(m, kWaveRows), (unsigned)n);
+}
+inline dim3 fused_block() { return dim3(64, kWaveRows); }
+
+}  // namespace bgk

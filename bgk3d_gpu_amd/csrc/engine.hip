@@ -1,0 +1,755 @@
+// Handle API (include/bgk3d_engine.h): library-owned lattices laid out for
+// gfx950, boundaries folded into the fused pull, FP32/FP64/FP16/BF16
+// storage.
+//
+// Layout of one lattice: 19 (or 18) arrays in one allocation.  Each array is
+// the reference's (0:nx-1, 0:ny-1, 0:nz-1) halo array with nx rounded up to a
+// multiple of A = 128 bytes / sizeof(storage) and the array start shifted by
+// A-1 elements, so the first interior cell of every row (i = 1) begins a
+// 128-byte line: a wave's loads of the unshifted directions and all its
+// stores then touch whole lines.  Direction 19 is updated in place and lives
+// only in lattice 0; directions 1..18 alternate between the two lattices
+// (the reference's pointer swap, SRC/col_MC.F90:539-596), tracked by `cur`.
+#include "../../include/bgk3d_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "collide.hpp"
+#include "common.hpp"
+#include "d3q19.hpp"
+#include "host_init.hpp"
+#include "reduce.hpp"
+#include "storage.hpp"
+#include "v1_kernels.hpp"
+
+using namespace bgk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define ENG_HIP(call)                                                                                    \
+    do {                                                                                                 \
+        hipError_t _e = (call);                                                                          \
+        if (_e != hipSuccess) return fail(BGK_EHIP, "%s: %s", #call, hipGetErrorString(_e));              \
+    } while (0)
+
+size_t storage_bytes(int kind)
+{
+    switch (kind) {
+        case BGK_F32: return 4;
+        case BGK_F64: return 8;
+        default: return 2;
+    }
+}
+
+long round_up(long v, long a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+struct bgk_handle {
+    bgk_config cfg{};
+    size_t esize = 4;
+    int A = 32;  // elements per 128-byte line
+    int nx = 0, ny = 0, nz = 0;
+    long nxy = 0, astride = 0, base = 0;
+    void* mem[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+    int cur = 0;
+    hipStream_t stream = nullptr;
+    double omega = 0, fgrad = 0, lidforce = 0, u_inflow = 0;
+    // obstacle
+    bool has_obs = false;
+    int ic = 0, jc = 0, kc = 0, radius = 0;
+    int ob[6] = {0, 0, 0, 0, 0, 0};  // link-mask box origin (i,j,k) and extent
+    uint32_t* obits = nullptr;
+    int mk[6] = {0, 0, 0, 0, 0, 0};  // solid-mask box origin and extent (draglift)
+    int8_t* omask = nullptr;
+    double* scratch = nullptr;
+    double* hscratch = nullptr;
+    long long steps = 0;
+    std::vector<hipEvent_t> events;
+
+    template <typename S>
+    S* arr(int buf, int d) const
+    {
+        return reinterpret_cast<S*>(mem[buf]) + (long)d * astride + base;
+    }
+    template <typename S>
+    S* cur_a(int d) const
+    {
+        return d == 18 ? arr<S>(0, 18) : arr<S>(cur, d);
+    }
+    template <typename S>
+    Lattice19<S> current() const
+    {
+        Lattice19<S> L;
+        for (int d = 0; d < 19; ++d) L.p[d] = cur_a<S>(d);
+        return L;
+    }
+    Geometry geom() const
+    {
+        Geometry g;
+        g.l = cfg.l;
+        g.m = cfg.m;
+        g.n = cfg.n;
+        g.nx = nx;
+        g.nxy = nxy;
+        return g;
+    }
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// kernels specific to the engine
+// ---------------------------------------------------------------------------
+
+template <typename S>
+__global__ void convert_in(const typename Storage<S>::compute* __restrict__ src, S* __restrict__ dst, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) store_value(dst + t, Rounder<S, 0>::cvt(src[t], 0u));
+}
+
+// Inflow halo x = 0, directions 1..5, every (j,k) incl. halo rows: the
+// shifted equilibrium for rho = 1, u = (u_inflow, 0, 0) of
+// SRC/bcond.inflow.F90:165-181 (cte1 = 0).  Constant in time, so it is
+// written once into both lattices.
+template <typename S>
+__global__ void inflow_halo(Lattice19<S> a, Geometry g, typename Storage<S>::compute u_inflow)
+{
+    typedef typename Storage<S>::compute R;
+    const int nj = g.m + 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nj * (g.n + 2)) return;
+    const int j = t % nj, k = t / nj;
+    const R p1 = W<R>::p1, p2 = W<R>::p2, rf = R(3), qf = R(1.5), three = R(3), crho = R(1), cte1 = R(0);
+    const R xj = u_inflow, yj = R(0), zj = R(0);
+    const R cvsq = xj * xj + yj * yj + zj * zj;
+    const R cx01 = rf * (xj - yj) + qf * (three * (xj - yj) * (xj - yj) - cvsq);
+    const R cx02 = rf * (xj - zj) + qf * (three * (xj - zj) * (xj - zj) - cvsq);
+    const R cx03 = rf * (xj + yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
+    const R cx04 = rf * (xj + zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
+    const R cx05 = rf * (xj) + qf * (three * (xj) * (xj)-cvsq);
+    const long s = gidx(g, 0, j, k);
+    store_value(a.p[0] + s, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx01), 0u));
+    store_value(a.p[1] + s, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx02), 0u));
+    store_value(a.p[2] + s, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx03), 0u));
+    store_value(a.p[3] + s, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx04), 0u));
+    store_value(a.p[4] + s, Rounder<S, 0>::cvt(crho * p1 * (cte1 + cx05), 0u));
+}
+
+// Outflow halo x = l+1, directions 10..14, interior (j,k): shifted
+// equilibrium with rho = 1 and the momentum of cell (l,j,k)
+// (SRC/bcond.inflow.F90:131-162).  The fused pull wraps j and k before
+// reading this halo, which reproduces the reference's y/z periodic copies of
+// the halo rows.
+template <typename S>
+__global__ void outflow_halo(Lattice19<S> a, Geometry g)
+{
+    typedef typename Storage<S>::compute R;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.m * g.n) return;
+    const int j = 1 + t % g.m, k = 1 + t / g.m, l = g.l;
+    const R p1 = W<R>::p1, p2 = W<R>::p2, rf = R(3), qf = R(1.5), three = R(3), crho = R(1), rhoinv = R(1),
+            cte1 = R(0);
+    const long si = gidx(g, l, j, k);
+    R f[19];
+#pragma unroll
+    for (int d = 0; d < 18; ++d) f[d] = Storage<S>::load(a.p[d] + si);
+    const R xj = +(f[0] + f[1] + f[2] + f[3] + f[4] - f[9] - f[10] - f[11] - f[12] - f[13]) * rhoinv;
+    const R yj = +(f[2] + f[6] + f[7] + f[8] + f[11] - f[0] - f[9] - f[15] - f[16] - f[17]) * rhoinv;
+    const R zj = +(f[3] + f[5] + f[6] + f[12] + f[17] - f[1] - f[8] - f[10] - f[14] - f[15]) * rhoinv;
+    const R cvsq = xj * xj + yj * yj + zj * zj;
+    const R cx10 = rf * (-xj - yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
+    const R cx11 = rf * (-xj - zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
+    const R cx12 = rf * (-xj + yj) + qf * (three * (xj - yj) * (xj - yj) - cvsq);
+    const R cx13 = rf * (-xj + zj) + qf * (three * (xj - zj) * (xj - zj) - cvsq);
+    const R cx14 = rf * (-xj) + qf * (three * (xj) * (xj)-cvsq);
+    const long so = gidx(g, l + 1, j, k);
+    store_value(a.p[9] + so, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx10), 0u));
+    store_value(a.p[10] + so, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx11), 0u));
+    store_value(a.p[11] + so, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx12), 0u));
+    store_value(a.p[12] + so, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx13), 0u));
+    store_value(a.p[13] + so, Rounder<S, 0>::cvt(crho * p1 * (cte1 + cx14), 0u));
+}
+
+// Momentum exchange over fluid cells of [is..ie]x[js..je]x[ks..ke]
+// (SRC/draglift.F90:77-176); the solid mask is held for a box around the
+// obstacle only (cells outside it are fluid).
+template <typename S>
+__global__ __launch_bounds__(kRedThreads) void draglift_box(Lattice19<S> a, Geometry g, const int8_t* __restrict__ om,
+                                                            int mi0, int mj0, int mk0, int mni, int mnj, int mnk,
+                                                            int is, int ie, int js, int je, int ks, int ke,
+                                                            double* __restrict__ part)
+{
+    typedef typename Storage<S>::compute R;
+    auto O = [&](int i, int j, int k) -> int {
+        const unsigned ri = (unsigned)(i - mi0), rj = (unsigned)(j - mj0), rk = (unsigned)(k - mk0);
+        if (ri >= (unsigned)mni || rj >= (unsigned)mnj || rk >= (unsigned)mnk) return 0;
+        return om[(long)ri + (long)mni * ((long)rj + (long)mnj * rk)];
+    };
+    double v[3] = {0, 0, 0};
+    const int ni = ie - is + 1, nj = je - js + 1, nk = ke - ks + 1;
+    const long cells = (long)ni * nj * nk;
+    for (long c = (long)blockIdx.x * kRedThreads + threadIdx.x; c < cells; c += (long)gridDim.x * kRedThreads) {
+        const int i = is + (int)(c % ni), j = js + (int)((c / ni) % nj), k = ks + (int)(c / ((long)ni * nj));
+        if (O(i, j, k) != 0) continue;
+        const long s = gidx(g, i, j, k);
+        R fxs = R(0), fys = R(0), fzs = R(0);
+#pragma unroll
+        for (int d = 0; d < 18; ++d) {
+            const R ad = Storage<S>::load(a.p[d] + s);
+            const R ob = (R)O(i + cx_of(d), j + cy_of(d), k + cz_of(d));
+            const R tx = R(2.0) * (R)cx_of(d) * ad * ob;
+            const R ty = R(2.0) * (R)cy_of(d) * ad * ob;
+            const R tz = R(2.0) * (R)cz_of(d) * ad * ob;
+            fxs = d == 0 ? tx : fxs + tx;
+            fys = d == 0 ? ty : fys + ty;
+            fzs = d == 0 ? tz : fzs + tz;
+        }
+        v[0] += (double)fxs;
+        v[1] += (double)fys;
+        v[2] += (double)fzs;
+    }
+    block_sum<3>(v);
+    if (threadIdx.x == 0) {
+        part[(long)blockIdx.x * 3 + 0] = v[0];
+        part[(long)blockIdx.x * 3 + 1] = v[1];
+        part[(long)blockIdx.x * 3 + 2] = v[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+
+bool sphere_solid(const bgk_handle& h, int i, int j, int k)
+{
+    if (!h.has_obs) return false;
+    if (i < 1 || i > h.cfg.l || j < 1 || j > h.cfg.m || k < 1 || k > h.cfg.n) return false;
+    const long d2 = (long)(h.ic - i) * (h.ic - i) + (long)(h.jc - j) * (h.jc - j) + (long)(h.kc - k) * (h.kc - k);
+    const long R = h.radius;
+    if (h.cfg.obstacle == 1) return d2 > (R - 2) * (R - 2) && d2 < (R + 2) * (R + 2);  // SRC/build.obs.F90:70-81
+    return d2 <= R * R;                                                                // solid sphere
+}
+
+int wrap(int v, int n) { return v < 1 ? v + n : (v > n ? v - n : v); }
+
+template <typename S>
+int upload_state(bgk_handle& h)
+{
+    typedef typename Storage<S>::compute C;
+    const bgk_config& c = h.cfg;
+    // rest state: shifted populations are all zero, which the memset left
+    if (c.init == 1) {
+        // Taylor-Green: host f_eq slab by slab (<= 16 planes of all 19 arrays)
+        const int slab = 16;
+        const long plane = h.nxy;
+        std::vector<std::vector<C>> host(19, std::vector<C>((size_t)(plane * slab), C(0)));
+        C* ptrs[19];
+        for (int d = 0; d < 19; ++d) ptrs[d] = host[d].data();
+        C* tmp = nullptr;
+        ENG_HIP(hipMalloc(&tmp, plane * slab * sizeof(C)));
+        const C u00 = (C)(c.boundary == BGK_BC_CHANNEL ? 0.0 : c.u0);
+        int rc = 0;
+        for (int k0 = 0; k0 <= c.n + 1 && rc == 0; k0 += slab) {
+            const int k1 = std::min(c.n + 1, k0 + slab - 1);
+            const long cnt = plane * (k1 - k0 + 1);
+            if (init_feq_host<C>(ptrs, c.l, c.m, c.n, h.nx, h.ny, c.init, u00, k0, k1) != 0) {
+                rc = fail(BGK_EINVAL, "init mode %d not supported", c.init);
+                break;
+            }
+            for (int d = 0; d < 19 && rc == 0; ++d) {
+                if (hipMemcpy(tmp, ptrs[d], cnt * sizeof(C), hipMemcpyHostToDevice) != hipSuccess) {
+                    rc = fail(BGK_EHIP, "upload of population %d failed", d);
+                    break;
+                }
+                for (int buf = 0; buf < 2; ++buf) {
+                    if (buf == 1 && d == 18) continue;
+                    hipLaunchKernelGGL(convert_in<S>, dim3(ceil_div(cnt, 256)), dim3(256), 0, h.stream, tmp,
+                                       h.arr<S>(buf, d) + plane * k0, cnt);
+                }
+                if (hipStreamSynchronize(h.stream) != hipSuccess) rc = fail(BGK_EHIP, "conversion failed");
+            }
+        }
+        hipFree(tmp);
+        if (rc) return rc;
+    } else if (c.init != 0) {
+        return fail(BGK_EINVAL, "init mode %d not supported", c.init);
+    }
+    if (c.boundary == BGK_BC_INFLOW) {
+        for (int buf = 0; buf < 2; ++buf) {
+            Lattice19<S> L;
+            for (int d = 0; d < 19; ++d) L.p[d] = d == 18 ? h.arr<S>(0, 18) : h.arr<S>(buf, d);
+            hipLaunchKernelGGL(inflow_halo<S>, dim3(ceil_div((long)(c.m + 2) * (c.n + 2), 256)), dim3(256), 0,
+                               h.stream, L, h.geom(), (C)h.u_inflow);
+        }
+        ENG_HIP(hipStreamSynchronize(h.stream));
+    }
+    return 0;
+}
+
+template <typename S, int BX, int BY, int BZ, bool OB, int FM, int SR>
+void launch_fused(bgk_handle& h)
+{
+    typedef typename Storage<S>::compute C;
+    FusedArgs<S> p{};
+    for (int d = 0; d < 19; ++d) p.a[d] = h.cur_a<S>(d);
+    for (int d = 0; d < 18; ++d) p.b[d] = h.arr<S>(1 - h.cur, d);
+    p.l = h.cfg.l;
+    p.m = h.cfg.m;
+    p.n = h.cfg.n;
+    p.nx = h.nx;
+    p.nxy = h.nxy;
+    p.omega = (C)h.omega;
+    p.fgrad = (C)h.fgrad;
+    p.fx = p.fy = p.fz = C(0);
+    p.lidforce = (C)h.lidforce;
+    p.cst = CollideConsts<C>::shifted();
+    p.obits = h.obits;
+    p.oi0 = h.ob[0];
+    p.oj0 = h.ob[1];
+    p.ok0 = h.ob[2];
+    p.oni = h.ob[3];
+    p.onj = h.ob[4];
+    p.onk = h.ob[5];
+    p.align = 0;
+    p.seed = hash32(h.cfg.seed ^ hash32((uint32_t)h.steps * 0x9e3779b9U + 1u));
+    hipLaunchKernelGGL((collide_fused<S, BX, BY, BZ, OB, FM, SR>), fused_grid(p.l, p.m, p.n, 0), fused_block(), 0,
+                       h.stream, p);
+}
+
+template <typename S, int SR>
+void dispatch_fused(bgk_handle& h)
+{
+    const bgk_config& c = h.cfg;
+    switch (c.boundary) {
+        case BGK_BC_DRIVEN: launch_fused<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>(h); break;
+        case BGK_BC_PERIODIC:
+            if (h.has_obs)
+                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_PERIODIC, true, 0, SR>(h);
+            else
+                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_PERIODIC, false, 0, SR>(h);
+            break;
+        case BGK_BC_CHANNEL:
+            if (c.force_axis == 2)
+                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 2, SR>(h);
+            else if (c.force_axis == 3)
+                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 3, SR>(h);
+            else
+                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 1, SR>(h);
+            break;
+        default:
+            if (h.has_obs)
+                launch_fused<S, AX_HALO, AX_PERIODIC, AX_PERIODIC, true, 0, SR>(h);
+            else
+                launch_fused<S, AX_HALO, AX_PERIODIC, AX_PERIODIC, false, 0, SR>(h);
+            break;
+    }
+}
+
+// queue the boundary work and the fused collide of one step, then swap
+template <typename S>
+void one_step(bgk_handle& h, hipEvent_t pre, hipEvent_t post)
+{
+    if (h.cfg.boundary == BGK_BC_INFLOW) {
+        hipLaunchKernelGGL(outflow_halo<S>, dim3(ceil_div((long)h.cfg.m * h.cfg.n, 256)), dim3(256), 0, h.stream,
+                           h.current<S>(), h.geom());
+    }
+    if (pre) hipEventRecord(pre, h.stream);
+    if (h.cfg.rounding && (h.cfg.storage == BGK_F16 || h.cfg.storage == BGK_BF16))
+        dispatch_fused<S, 1>(h);
+    else
+        dispatch_fused<S, 0>(h);
+    if (post) hipEventRecord(post, h.stream);
+    h.cur = 1 - h.cur;
+    ++h.steps;
+}
+
+template <typename F>
+int by_storage(int kind, F&& f)
+{
+    switch (kind) {
+        case BGK_F32: return f(float{});
+        case BGK_F64: return f(double{});
+        case BGK_F16: return f(f16_t{});
+        case BGK_BF16: return f(bf16_t{});
+    }
+    return fail(BGK_EINVAL, "unknown storage type %d", kind);
+}
+
+template <typename S>
+typename Storage<S>::compute cte_one()
+{
+    return typename Storage<S>::compute(1);
+}
+
+}  // namespace
+
+extern "C" {
+
+void bgk_config_default(bgk_config* c, int l, int m, int n)
+{
+    std::memset(c, 0, sizeof *c);
+    c->l = l;
+    c->m = m;
+    c->n = n;
+    c->boundary = BGK_BC_DRIVEN;
+    c->storage = BGK_F32;
+    c->svisc = 0.064;
+    c->u0 = 0.1;
+    c->obs_ic = c->obs_jc = c->obs_kc = c->obs_radius = -1;
+    c->u_inflow = -1.0;
+    c->force_axis = 1;
+}
+
+const char* bgk_last_error(void) { return g_err.c_str(); }
+
+int bgk_create(const bgk_config* cfg, bgk_handle** out)
+{
+    if (!cfg || !out) return fail(BGK_EINVAL, "null argument");
+    *out = nullptr;
+    const bgk_config& c = *cfg;
+    if (c.l < 1 || c.m < 1 || c.n < 1) return fail(BGK_EINVAL, "grid %d x %d x %d", c.l, c.m, c.n);
+    if (c.boundary < 0 || c.boundary > 3) return fail(BGK_EINVAL, "boundary %d", c.boundary);
+    if (c.storage < 0 || c.storage > 3) return fail(BGK_EINVAL, "storage %d", c.storage);
+    if (c.init != 0 && c.init != 1) return fail(BGK_EINVAL, "init %d", c.init);
+    if (c.obstacle < 0 || c.obstacle > 2) return fail(BGK_EINVAL, "obstacle %d", c.obstacle);
+    if (c.obstacle && c.boundary != BGK_BC_INFLOW && c.boundary != BGK_BC_PERIODIC)
+        return fail(BGK_EINVAL, "obstacles are supported with inflow or periodic boundaries");
+    if (!(c.svisc > 0.0)) return fail(BGK_EINVAL, "svisc must be positive");
+    if (c.force_axis < 1 || c.force_axis > 3) return fail(BGK_EINVAL, "force_axis %d", c.force_axis);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || c.device < 0 || c.device >= ndev)
+        return fail(BGK_EINVAL, "device %d not available (%d devices)", c.device, ndev);
+    ENG_HIP(hipSetDevice(c.device));
+
+    bgk_handle* h = new bgk_handle();
+    h->cfg = c;
+    h->esize = storage_bytes(c.storage);
+    h->A = (int)(128 / h->esize);
+    h->nx = (int)round_up(c.l + 2, h->A);
+    h->ny = c.m + 2;
+    h->nz = c.n + 2;
+    h->nxy = (long)h->nx * h->ny;
+    h->base = h->A - 1;
+    h->astride = round_up(h->A + h->nxy * h->nz, h->A);
+    // derived parameters in the collision kind (SRC/hencol.F90:31, :52;
+    // SRC/bcond.driven.F90:61; SRC/build.bcond.F90:29-35)
+    const bool dbl = c.storage == BGK_F64;
+    auto K = [&](double v) { return dbl ? v : (double)(float)v; };
+    const double nu = K(c.svisc), u0f = K(c.boundary == BGK_BC_CHANNEL ? c.u0 : 0.0),
+                 u00 = K(c.boundary == BGK_BC_CHANNEL ? 0.0 : c.u0);
+    if (dbl) {
+        h->omega = (1.0 + 1.0) / ((6.0 * nu) + 1.0);
+        h->fgrad = 4.0 * u0f * nu / (5.0 * (double)c.m * (double)c.m);
+        h->lidforce = u00 / 6.0;
+    } else {
+        const float nuf = (float)nu;
+        h->omega = (1.0f + 1.0f) / ((6.0f * nuf) + 1.0f);
+        h->fgrad = 4.0f * (float)u0f * nuf / (5.0f * (float)c.m * (float)c.m);
+        h->lidforce = (float)u00 / 6.0f;
+    }
+    h->u_inflow = c.u_inflow < 0 ? (double)0.1f : K(c.u_inflow);
+
+    int rc = 0;
+    auto cleanup = [&](int code) {
+        bgk_destroy(h);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(BGK_EHIP, "stream creation failed"));
+    const size_t bytes0 = (size_t)h->astride * 19 * h->esize, bytes1 = (size_t)h->astride * 18 * h->esize;
+    if (hipMalloc(&h->mem[0], bytes0) != hipSuccess || hipMalloc(&h->mem[1], bytes1) != hipSuccess)
+        return cleanup(fail(BGK_ENOMEM, "cannot allocate %.2f GB of lattice", (bytes0 + bytes1) / 1e9));
+    h->bytes = bytes0 + bytes1;
+    hipMemsetAsync(h->mem[0], 0, bytes0, h->stream);
+    hipMemsetAsync(h->mem[1], 0, bytes1, h->stream);
+
+    // obstacle: reference sphere shell by default (SRC/build.obs.F90:52-60)
+    if (c.obstacle) {
+        h->has_obs = true;
+        h->ic = c.obs_ic >= 0 ? c.obs_ic : 2 * c.l / 5;
+        h->jc = c.obs_jc >= 0 ? c.obs_jc : c.m / 2;
+        h->kc = c.obs_kc >= 0 ? c.obs_kc : c.n / 2;
+        h->radius = c.obs_radius >= 0 ? c.obs_radius : (c.m > 256 ? 32 : c.m / 8);
+        const int R = h->radius + 3;
+        // link masks for every cell within R+1 of the centre (bit d: x - c_d solid)
+        int lo[3] = {std::max(1, h->ic - R), std::max(1, h->jc - R), std::max(1, h->kc - R)};
+        int hi[3] = {std::min(c.l, h->ic + R), std::min(c.m, h->jc + R), std::min(c.n, h->kc + R)};
+        const int ni = hi[0] - lo[0] + 1, nj = hi[1] - lo[1] + 1, nk = hi[2] - lo[2] + 1;
+        std::vector<uint32_t> bits((size_t)ni * nj * nk, 0u);
+        const bool px = c.boundary == BGK_BC_PERIODIC;
+        for (int k = lo[2]; k <= hi[2]; ++k)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+                for (int i = lo[0]; i <= hi[0]; ++i) {
+                    uint32_t b = 0;
+                    for (int d = 0; d < 18; ++d) {
+                        int si = i - cx_of(d), sj = wrap(j - cy_of(d), c.m), sk = wrap(k - cz_of(d), c.n);
+                        if (px) si = wrap(si, c.l);
+                        if (sphere_solid(*h, si, sj, sk)) b |= 1u << d;
+                    }
+                    bits[(size_t)(i - lo[0]) + (size_t)ni * ((j - lo[1]) + (size_t)nj * (k - lo[2]))] = b;
+                }
+        h->ob[0] = lo[0];
+        h->ob[1] = lo[1];
+        h->ob[2] = lo[2];
+        h->ob[3] = ni;
+        h->ob[4] = nj;
+        h->ob[5] = nk;
+        // solid mask over centre +- (radius + 4) for the drag/lift sums
+        const int Q = h->radius + 4;
+        int mlo[3] = {std::max(1, h->ic - Q), std::max(1, h->jc - Q), std::max(1, h->kc - Q)};
+        int mhi[3] = {std::min(c.l, h->ic + Q), std::min(c.m, h->jc + Q), std::min(c.n, h->kc + Q)};
+        const int mi = mhi[0] - mlo[0] + 1, mj = mhi[1] - mlo[1] + 1, mkk = mhi[2] - mlo[2] + 1;
+        std::vector<int8_t> mask((size_t)mi * mj * mkk, 0);
+        for (int k = mlo[2]; k <= mhi[2]; ++k)
+            for (int j = mlo[1]; j <= mhi[1]; ++j)
+                for (int i = mlo[0]; i <= mhi[0]; ++i)
+                    mask[(size_t)(i - mlo[0]) + (size_t)mi * ((j - mlo[1]) + (size_t)mj * (k - mlo[2]))] =
+                        sphere_solid(*h, i, j, k) ? 1 : 0;
+        h->mk[0] = mlo[0];
+        h->mk[1] = mlo[1];
+        h->mk[2] = mlo[2];
+        h->mk[3] = mi;
+        h->mk[4] = mj;
+        h->mk[5] = mkk;
+        if (hipMalloc(&h->obits, bits.size() * 4) != hipSuccess || hipMalloc(&h->omask, mask.size()) != hipSuccess)
+            return cleanup(fail(BGK_ENOMEM, "obstacle masks"));
+        if (hipMemcpy(h->obits, bits.data(), bits.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->omask, mask.data(), mask.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(BGK_EHIP, "obstacle upload"));
+    }
+    if (hipMalloc(&h->scratch, (kRedBlocks + 8) * 5 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&h->hscratch, (kRedBlocks + 8) * 5 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(BGK_ENOMEM, "scratch"));
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(fail(BGK_EHIP, "memset"));
+    rc = by_storage(c.storage, [&](auto tag) { return upload_state<decltype(tag)>(*h); });
+    if (rc) return cleanup(rc);
+    *out = h;
+    return 0;
+}
+
+int bgk_destroy(bgk_handle* h)
+{
+    if (!h) return 0;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (auto e : h->events) hipEventDestroy(e);
+    if (h->mem[0]) hipFree(h->mem[0]);
+    if (h->mem[1]) hipFree(h->mem[1]);
+    if (h->obits) hipFree(h->obits);
+    if (h->omask) hipFree(h->omask);
+    if (h->scratch) hipFree(h->scratch);
+    if (h->hscratch) hipHostFree(h->hscratch);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+int bgk_step(bgk_handle* h, int nsteps)
+{
+    if (!h || nsteps < 0) return fail(BGK_EINVAL, "bad arguments");
+    ENG_HIP(hipSetDevice(h->cfg.device));
+    return by_storage(h->cfg.storage, [&](auto tag) {
+        for (int s = 0; s < nsteps; ++s) one_step<decltype(tag)>(*h, nullptr, nullptr);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? 0 : fail(BGK_EHIP, "step launch: %s", hipGetErrorString(e));
+    });
+}
+
+int bgk_sync(bgk_handle* h)
+{
+    if (!h) return fail(BGK_EINVAL, "null handle");
+    ENG_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int bgk_time_steps(bgk_handle* h, int nsteps, double* total_ms, double* collide_avg_ms)
+{
+    if (!h || nsteps < 1) return fail(BGK_EINVAL, "bad arguments");
+    ENG_HIP(hipSetDevice(h->cfg.device));
+    const size_t need = 2 * (size_t)nsteps + 2;
+    while (h->events.size() < need) {
+        hipEvent_t e;
+        ENG_HIP(hipEventCreate(&e));
+        h->events.push_back(e);
+    }
+    ENG_HIP(hipStreamSynchronize(h->stream));
+    hipEvent_t e0 = h->events[0], e1 = h->events[1];
+    ENG_HIP(hipEventRecord(e0, h->stream));
+    int rc = by_storage(h->cfg.storage, [&](auto tag) {
+        for (int s = 0; s < nsteps; ++s)
+            one_step<decltype(tag)>(*h, h->events[2 + 2 * s], h->events[3 + 2 * s]);
+        return 0;
+    });
+    if (rc) return rc;
+    ENG_HIP(hipEventRecord(e1, h->stream));
+    ENG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    ENG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    double ksum = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        float k = 0;
+        ENG_HIP(hipEventElapsedTime(&k, h->events[2 + 2 * s], h->events[3 + 2 * s]));
+        ksum += k;
+    }
+    if (total_ms) *total_ms = ms;
+    if (collide_avg_ms) *collide_avg_ms = ksum / nsteps;
+    return 0;
+}
+
+int bgk_diagno(bgk_handle* h, double out[5])
+{
+    if (!h || !out) return fail(BGK_EINVAL, "bad arguments");
+    ENG_HIP(hipSetDevice(h->cfg.device));
+    const long rows = (long)h->cfg.m * h->cfg.n;
+    const int nb = (int)std::min<long>(rows, kRedBlocks);
+    int rc = by_storage(h->cfg.storage, [&](auto tag) {
+        typedef decltype(tag) S;
+        hipLaunchKernelGGL(diagno_partial<S>, dim3(nb), dim3(kRedThreads), 0, h->stream, h->current<S>(), h->geom(),
+                           cte_one<S>(), h->scratch + 5);
+        hipLaunchKernelGGL(fold_partials<5>, dim3(1), dim3(kRedThreads), 0, h->stream, h->scratch + 5, nb,
+                           h->scratch);
+        return 0;
+    });
+    if (rc) return rc;
+    ENG_HIP(hipMemcpyAsync(h->hscratch, h->scratch, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ENG_HIP(hipStreamSynchronize(h->stream));
+    const double fl = (double)(float)h->cfg.l, fm = (double)(float)h->cfg.m, fn = (double)(float)h->cfg.n;
+    for (int q = 0; q < 5; ++q) out[q] = (h->hscratch[q] / fl) / fm / fn;
+    return 0;
+}
+
+int bgk_draglift(bgk_handle* h, double out[3])
+{
+    if (!h || !out) return fail(BGK_EINVAL, "bad arguments");
+    if (!h->has_obs) return fail(BGK_EINVAL, "no obstacle");
+    ENG_HIP(hipSetDevice(h->cfg.device));
+    const bgk_config& c = h->cfg;
+    const int delta = 3;  // SRC/draglift.F90:54
+    const int is = std::max(1, h->ic - h->radius - delta), ie = std::min(c.l, h->ic + h->radius + delta);
+    const int js = std::max(1, h->jc - h->radius - delta), je = std::min(c.m, h->jc + h->radius + delta);
+    const int ks = std::max(1, h->kc - h->radius - delta), ke = std::min(c.n, h->kc + h->radius + delta);
+    const long cells = (long)(ie - is + 1) * (je - js + 1) * (ke - ks + 1);
+    const int nb = (int)std::min<long>((cells + kRedThreads - 1) / kRedThreads, kRedBlocks);
+    by_storage(c.storage, [&](auto tag) {
+        typedef decltype(tag) S;
+        hipLaunchKernelGGL(draglift_box<S>, dim3(nb), dim3(kRedThreads), 0, h->stream, h->current<S>(), h->geom(),
+                           h->omask, h->mk[0], h->mk[1], h->mk[2], h->mk[3], h->mk[4], h->mk[5], is, ie, js, je, ks,
+                           ke, h->scratch + 5);
+        hipLaunchKernelGGL(fold_partials<3>, dim3(1), dim3(kRedThreads), 0, h->stream, h->scratch + 5, nb,
+                           h->scratch);
+        return 0;
+    });
+    ENG_HIP(hipMemcpyAsync(h->hscratch, h->scratch, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ENG_HIP(hipStreamSynchronize(h->stream));
+    // norm = uno/(u_inflow*u_inflow*radius) in the working kind (SRC/draglift.F90:178)
+    double norm;
+    if (c.storage == BGK_F64)
+        norm = 1.0 / (h->u_inflow * h->u_inflow * (double)h->radius);
+    else {
+        const float u = (float)h->u_inflow;
+        norm = (double)(1.0f / (u * u * (float)h->radius));
+    }
+    for (int q = 0; q < 3; ++q) out[q] = h->hscratch[q] * norm;
+    return 0;
+}
+
+int bgk_fields(bgk_handle* h, double* rho, double* ux, double* uy, double* uz)
+{
+    if (!h) return fail(BGK_EINVAL, "null handle");
+    ENG_HIP(hipSetDevice(h->cfg.device));
+    const long cells = (long)h->cfg.l * h->cfg.m * h->cfg.n;
+    double* d = nullptr;
+    ENG_HIP(hipMalloc(&d, 4 * cells * sizeof(double)));
+    by_storage(h->cfg.storage, [&](auto tag) {
+        typedef decltype(tag) S;
+        hipLaunchKernelGGL(fields_kernel<S>, dim3(ceil_div(cells, 256)), dim3(256), 0, h->stream, h->current<S>(),
+                           h->geom(), cte_one<S>(), d, d + cells, d + 2 * cells, d + 3 * cells);
+        return 0;
+    });
+    double* outs[4] = {rho, ux, uy, uz};
+    int rc = 0;
+    for (int q = 0; q < 4 && rc == 0; ++q)
+        if (outs[q] && hipMemcpyAsync(outs[q], d + q * cells, cells * sizeof(double), hipMemcpyDeviceToHost,
+                                      h->stream) != hipSuccess)
+            rc = fail(BGK_EHIP, "field download");
+    if (hipStreamSynchronize(h->stream) != hipSuccess && rc == 0) rc = fail(BGK_EHIP, "fields");
+    hipFree(d);
+    return rc;
+}
+
+int bgk_get_population(bgk_handle* h, int dir, double* out)
+{
+    if (!h || !out || dir < 0 || dir > 18) return fail(BGK_EINVAL, "bad arguments");
+    ENG_HIP(hipSetDevice(h->cfg.device));
+    const long cells = (long)h->cfg.l * h->cfg.m * h->cfg.n;
+    double* d = nullptr;
+    ENG_HIP(hipMalloc(&d, cells * sizeof(double)));
+    by_storage(h->cfg.storage, [&](auto tag) {
+        typedef decltype(tag) S;
+        hipLaunchKernelGGL(get_interior<S>, dim3(ceil_div(cells, 256)), dim3(256), 0, h->stream, h->cur_a<S>(dir),
+                           h->geom(), d);
+        return 0;
+    });
+    int rc = 0;
+    if (hipMemcpyAsync(out, d, cells * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        rc = fail(BGK_EHIP, "population download");
+    hipFree(d);
+    return rc;
+}
+
+int bgk_set_population(bgk_handle* h, int dir, const double* in)
+{
+    if (!h || !in || dir < 0 || dir > 18) return fail(BGK_EINVAL, "bad arguments");
+    ENG_HIP(hipSetDevice(h->cfg.device));
+    const long cells = (long)h->cfg.l * h->cfg.m * h->cfg.n;
+    double* d = nullptr;
+    ENG_HIP(hipMalloc(&d, cells * sizeof(double)));
+    int rc = 0;
+    if (hipMemcpyAsync(d, in, cells * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        rc = fail(BGK_EHIP, "population upload");
+    if (rc == 0)
+        by_storage(h->cfg.storage, [&](auto tag) {
+            typedef decltype(tag) S;
+            hipLaunchKernelGGL(set_interior<S>, dim3(ceil_div(cells, 256)), dim3(256), 0, h->stream, h->cur_a<S>(dir),
+                               h->geom(), d);
+            return 0;
+        });
+    if (hipStreamSynchronize(h->stream) != hipSuccess && rc == 0) rc = fail(BGK_EHIP, "population set");
+    hipFree(d);
+    return rc;
+}
+
+int bgk_info(bgk_handle* h, long long out[6])
+{
+    if (!h || !out) return fail(BGK_EINVAL, "bad arguments");
+    out[0] = h->nx;
+    out[1] = h->ny;
+    out[2] = h->nz;
+    out[3] = (long long)h->bytes;
+    out[4] = (long long)h->esize;
+    out[5] = h->steps;
+    return 0;
+}
+
+}  // extern "C"

@@ -12,6 +12,7 @@
 
 #include "common.hpp"
 #include "d3q19.hpp"
+#include "storage.hpp"
 #include "v1_kernels.hpp"
 
 namespace bgk {
@@ -72,10 +73,12 @@ __global__ __launch_bounds__(kRedThreads) void fold_partials(const double* __res
 // diagno (SRC/diagno.F90:89-131): per interior cell rho, u, v, w and
 // 0.5|u|^2 in the storage kind; partial sums {u, v, w, rho, ke} in FP64.
 // Rows (j,k) are distributed over blocks; lanes walk i.
-template <typename R>
-__global__ __launch_bounds__(kRedThreads) void diagno_partial(Lattice19<R> a, Geometry g, R cte1,
+template <typename S>
+__global__ __launch_bounds__(kRedThreads) void diagno_partial(Lattice19<S> a, Geometry g,
+                                                              typename Storage<S>::compute cte1,
                                                               double* __restrict__ part)
 {
+    typedef typename Storage<S>::compute R;
     double v[5] = {0, 0, 0, 0, 0};
     const long rows = (long)g.m * g.n;
     for (long r = blockIdx.x; r < rows; r += gridDim.x) {
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(kRedThreads) void diagno_partial(Lattice19<R> a, Ge
             const long s = gidx(g, i, j, k);
             R x[19];
 #pragma unroll
-            for (int d = 0; d < 19; ++d) x[d] = a.p[d][s];
+            for (int d = 0; d < 19; ++d) x[d] = Storage<S>::load(a.p[d] + s);
             const R rho = density_ref(x, cte1);
             const R rhoinv = R(1) / rho;
             const R xj = (x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) * rhoinv;
@@ -106,10 +109,12 @@ __global__ __launch_bounds__(kRedThreads) void diagno_partial(Lattice19<R> a, Ge
 
 // varm (SRC/varm.F90:56-92): velocity sums per z-plane k (block = plane)
 // -> out[3*(k-1) + {0,1,2}] = {u, w(y-vel), v(z-vel)}.
-template <typename R>
-__global__ __launch_bounds__(kRedThreads) void varm_planes_k(Lattice19<R> a, Geometry g, R cte1,
+template <typename S>
+__global__ __launch_bounds__(kRedThreads) void varm_planes_k(Lattice19<S> a, Geometry g,
+                                                             typename Storage<S>::compute cte1,
                                                              double* __restrict__ out)
 {
+    typedef typename Storage<S>::compute R;
     const int k = 1 + blockIdx.x;
     double v[3] = {0, 0, 0};
     const long cells = (long)g.l * g.m;
@@ -118,7 +123,7 @@ __global__ __launch_bounds__(kRedThreads) void varm_planes_k(Lattice19<R> a, Geo
         const long s = gidx(g, i, j, k);
         R x[19];
 #pragma unroll
-        for (int d = 0; d < 19; ++d) x[d] = a.p[d][s];
+        for (int d = 0; d < 19; ++d) x[d] = Storage<S>::load(a.p[d] + s);
         const R rhoinv = R(1) / density_ref(x, cte1);
         v[0] += (double)((x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) * rhoinv);
         v[1] += (double)((x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]) * rhoinv);
@@ -133,10 +138,12 @@ __global__ __launch_bounds__(kRedThreads) void varm_planes_k(Lattice19<R> a, Geo
 }
 
 // varm density sums per y-plane j (block = plane) -> out[j-1].
-template <typename R>
-__global__ __launch_bounds__(kRedThreads) void varm_planes_j(Lattice19<R> a, Geometry g, R cte1,
+template <typename S>
+__global__ __launch_bounds__(kRedThreads) void varm_planes_j(Lattice19<S> a, Geometry g,
+                                                             typename Storage<S>::compute cte1,
                                                              double* __restrict__ out)
 {
+    typedef typename Storage<S>::compute R;
     const int j = 1 + blockIdx.x;
     double v[1] = {0};
     const long cells = (long)g.l * g.n;
@@ -145,7 +152,7 @@ __global__ __launch_bounds__(kRedThreads) void varm_planes_j(Lattice19<R> a, Geo
         const long s = gidx(g, i, j, k);
         R x[19];
 #pragma unroll
-        for (int d = 0; d < 19; ++d) x[d] = a.p[d][s];
+        for (int d = 0; d < 19; ++d) x[d] = Storage<S>::load(a.p[d] + s);
         v[0] += (double)density_ref(x, cte1);
     }
     block_sum<1>(v);
@@ -155,12 +162,13 @@ __global__ __launch_bounds__(kRedThreads) void varm_planes_j(Lattice19<R> a, Geo
 // draglift (SRC/draglift.F90:77-176): momentum exchange over fluid cells of
 // the box, F = sum_d 2 c_d a_d(x) obs(x + c_d); terms in the storage kind as
 // the reference forms them, accumulation in FP64.
-template <typename R>
-__global__ __launch_bounds__(kRedThreads) void draglift_partial(Lattice19<R> a, Geometry g,
+template <typename S>
+__global__ __launch_bounds__(kRedThreads) void draglift_partial(Lattice19<S> a, Geometry g,
                                                                 const int* __restrict__ obs, int is, int ie,
                                                                 int js, int je, int ks, int ke,
                                                                 double* __restrict__ part)
 {
+    typedef typename Storage<S>::compute R;
     double v[3] = {0, 0, 0};
     const int ni = ie - is + 1, nj = je - js + 1, nk = ke - ks + 1;
     const long cells = (long)ni * nj * nk;
@@ -175,7 +183,7 @@ __global__ __launch_bounds__(kRedThreads) void draglift_partial(Lattice19<R> a, 
         bool first = true;
 #pragma unroll
         for (int d = 0; d < 18; ++d) {
-            const R ad = a.p[d][s];
+            const R ad = Storage<S>::load(a.p[d] + s);
             const R ob = (R)O(i + cx_of(d), j + cy_of(d), k + cz_of(d));
             const R tx = R(2.0) * (R)cx_of(d) * ad * ob;
             const R ty = R(2.0) * (R)cy_of(d) * ad * ob;
@@ -205,20 +213,66 @@ __global__ __launch_bounds__(kRedThreads) void draglift_partial(Lattice19<R> a, 
 
 // prof_k (SRC/prof_k.F90:47-79): line (ic, jc, 1..n), one thread per k.
 // out[4*(k-1) + {0,1,2,3}] = {u, w(y-vel), v(z-vel), den} in the storage kind.
-template <typename R>
-__global__ void prof_k_kernel(Lattice19<R> a, Geometry g, int ic, int jc, R cte1, R* __restrict__ out)
+template <typename S>
+__global__ void prof_k_kernel(Lattice19<S> a, Geometry g, int ic, int jc, typename Storage<S>::compute cte1,
+                              typename Storage<S>::compute* __restrict__ out)
 {
+    typedef typename Storage<S>::compute R;
     const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     if (k > g.n) return;
     const long s = gidx(g, ic, jc, k);
     R x[19];
 #pragma unroll
-    for (int d = 0; d < 19; ++d) x[d] = a.p[d][s];
+    for (int d = 0; d < 19; ++d) x[d] = Storage<S>::load(a.p[d] + s);
     const R den = density_ref(x, cte1);
     out[4 * (k - 1) + 0] = (x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) / den;
     out[4 * (k - 1) + 1] = (x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]) / den;
     out[4 * (k - 1) + 2] = (x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]) / den;
     out[4 * (k - 1) + 3] = den;
+}
+
+// Interior macroscopic fields (rho, ux, uy, uz) in FP64, i fastest; the
+// moments are formed in the collision kind with the reference's order.
+template <typename S>
+__global__ void fields_kernel(Lattice19<S> a, Geometry g, typename Storage<S>::compute cte1,
+                              double* __restrict__ rho, double* __restrict__ ux, double* __restrict__ uy,
+                              double* __restrict__ uz)
+{
+    typedef typename Storage<S>::compute R;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long cells = (long)g.l * g.m * g.n;
+    if (t >= cells) return;
+    const int i = 1 + (int)(t % g.l), j = 1 + (int)((t / g.l) % g.m), k = 1 + (int)(t / ((long)g.l * g.m));
+    const long s = gidx(g, i, j, k);
+    R x[19];
+#pragma unroll
+    for (int d = 0; d < 19; ++d) x[d] = Storage<S>::load(a.p[d] + s);
+    const R r = density_ref(x, cte1);
+    const R rinv = R(1) / r;
+    if (rho) rho[t] = (double)r;
+    if (ux) ux[t] = (double)((x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) * rinv);
+    if (uy) uy[t] = (double)((x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]) * rinv);
+    if (uz) uz[t] = (double)((x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]) * rinv);
+}
+
+// Interior of one population array <-> FP64 buffer (i fastest).
+template <typename S>
+__global__ void get_interior(const S* __restrict__ a, Geometry g, double* __restrict__ out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.l * g.m * g.n) return;
+    const int i = 1 + (int)(t % g.l), j = 1 + (int)((t / g.l) % g.m), k = 1 + (int)(t / ((long)g.l * g.m));
+    out[t] = (double)Storage<S>::load(a + gidx(g, i, j, k));
+}
+
+template <typename S>
+__global__ void set_interior(S* __restrict__ a, Geometry g, const double* __restrict__ in)
+{
+    typedef typename Storage<S>::compute R;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.l * g.m * g.n) return;
+    const int i = 1 + (int)(t % g.l), j = 1 + (int)((t / g.l) % g.m), k = 1 + (int)(t / ((long)g.l * g.m));
+    store_value(a + gidx(g, i, j, k), Rounder<S, 0>::cvt((R)in[t], 0u));
 }
 
 }  // namespace bgk

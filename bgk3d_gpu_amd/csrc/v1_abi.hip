@@ -11,13 +11,16 @@
 // never allocate it, never free it and never keep its pointers across calls
 // (the caller swaps a and b every step).  The diagnostics keep a small cached
 // device scratch (partial sums) and pinned host buffer.
+#include "collide.hpp"
 #include "common.hpp"
 #include "d3q19.hpp"
 #include "reduce.hpp"
 #include "v1_kernels.hpp"
+#include "host_init.hpp"
 
 #include "../../include/bgk3d_gpu.h"
 
+#include <algorithm>
 #include <cmath>
 
 #ifdef BGK_DOUBLE
@@ -66,16 +69,42 @@ void launch_face(K kern, long count, const char* name, Args... args)
     report_launch(name);
 }
 
+// The fused kernel in halo mode: every pull reads the caller's halo, which
+// the bcond_*_gpu entry points filled.  Waves are aligned to 128-byte lines
+// from the output addresses, whatever padding (ipad) the caller chose.
 template <int FM>
 void launch_col(const Lattice19<real_t>& A, const Lattice19<real_t>& B, const Geometry& g, real_t omega,
                 real_t fgrad, real_t ux, real_t uy, real_t uz, const CollideConsts<real_t>& c)
 {
     if (g.l <= 0 || g.m <= 0 || g.n <= 0) return;
-    dim3 grid(ceil_div(g.l, kColBX), ceil_div(g.m, kColBY), g.n);
-    dim3 block(kColBX, kColBY);
-    hipLaunchKernelGGL((col_mc_kernel<real_t, FM>), grid, block, 0, abi_stream(), A, B, g, omega, fgrad, ux, uy,
-                       uz, c);
-    report_launch("col_mc_kernel");
+    FusedArgs<real_t> p{};
+    for (int d = 0; d < 19; ++d) p.a[d] = A.p[d];
+    for (int d = 0; d < 18; ++d) p.b[d] = B.p[d];
+    p.l = g.l;
+    p.m = g.m;
+    p.n = g.n;
+    p.nx = g.nx;
+    p.nxy = g.nxy;
+    p.omega = omega;
+    p.fgrad = fgrad;
+    p.fx = ux;
+    p.fy = uy;
+    p.fz = uz;
+    p.lidforce = 0;
+    p.cst = c;
+    p.obits = nullptr;
+    p.align = (int)(128 / sizeof(real_t));
+    p.seed = 0;
+    // when every row shares one alignment the exact shift sizes the grid
+    int span = g.l + p.align - 1;
+    if (g.nx % p.align == 0 && g.nxy % p.align == 0) {
+        const long first = (long)(reinterpret_cast<uintptr_t>(p.b[0]) / sizeof(real_t)) + 1;
+        span = g.l + (int)(first & (p.align - 1));
+    }
+    dim3 grid((unsigned)ceil_div(span, 64), (unsigned)ceil_div(g.m, kWaveRows), (unsigned)g.n);
+    hipLaunchKernelGGL((collide_fused<real_t, AX_HALO, AX_HALO, AX_HALO, false, FM, 0>), grid, fused_block(), 0,
+                       abi_stream(), p);
+    report_launch("collide_fused");
 }
 
 // Cached scratch for the reductions (per device; single host thread, as in
@@ -320,53 +349,10 @@ int prof_k_gpu(real_t** a, int l, int m, int n, int nx, int ny, int icoord, int 
     return 0;
 }
 
-// Initial populations f = f_eq written into HOST arrays h[0..18] over every
-// cell including the halo (0..l+1, 0..m+1, 0..n+1), shifted storage
-// (SRC/init.F90:31-209; cte1 = 0 there).  mode 0: rest (all zero);
-// mode 1: 3-D Taylor-Green of SRC/init.F90:142-148.  Runs on the host as in
-// the reference, with the C library's sin/cos of the working kind, so the
-// state is bit-identical to the reference's.  Mixed-kind details kept:
-// pi is a default-real literal (:63) and the coordinates are formed in
-// double from (real(i,mykind)-0.5d0)/real(l,mykind) (:115-123).
+// Initial populations (host-side, as the reference's init), see host_init.hpp.
 int bgk_init_host(real_t** h, int l, int m, int n, int nx, int ny, int mode, real_t u00)
 {
-    if (mode != 0 && mode != 1) return -1;
-    const size_t nxy = (size_t)nx * (size_t)ny;
-    const real_t uno = 1, due = 2, tre = 3, qua = 4, zero = 0, rf = 3, qf = real_t(1.5), cte1 = 0;
-    const real_t pi = (real_t)3.141592653589793238462643383279f;
-    const real_t w[19] = {W<real_t>::p2, W<real_t>::p2, W<real_t>::p2, W<real_t>::p2, W<real_t>::p1,
-                          W<real_t>::p1, W<real_t>::p2, W<real_t>::p1, W<real_t>::p2, W<real_t>::p2,
-                          W<real_t>::p2, W<real_t>::p2, W<real_t>::p2, W<real_t>::p1, W<real_t>::p1,
-                          W<real_t>::p2, W<real_t>::p1, W<real_t>::p2, W<real_t>::p0};
-    for (int k = 0; k <= n + 1; ++k) {
-        const real_t z = (real_t)(((double)(real_t)k - 0.5) / (double)(real_t)n);
-        for (int j = 0; j <= m + 1; ++j) {
-            const real_t y = (real_t)(((double)(real_t)j - 0.5) / (double)(real_t)m);
-            for (int i = 0; i <= l + 1; ++i) {
-                const real_t x = (real_t)(((double)(real_t)i - 0.5) / (double)(real_t)l);
-                real_t xj = zero, yj = zero, zj = zero, crho = uno;
-                if (mode == 1) {
-                    xj = u00 * std::sin(due * pi * x) * std::cos(due * pi * y) * std::cos(due * pi * z);
-                    yj = -u00 * std::cos(due * pi * x) * std::sin(due * pi * y) * std::cos(due * pi * z);
-                    crho = uno + ((u00 * u00) / (qua * qua * tre)) * (std::cos(qua * pi * x) + std::cos(qua * pi * y)) *
-                                     (std::cos(qua * pi * z) + due);
-                }
-                const real_t cvsq = xj * xj + yj * yj + zj * zj;
-                // c.u per direction in the reference's grouping (SRC/init.F90:159-177)
-                const real_t lin[19] = {xj - yj, xj - zj, xj + yj, xj + zj, xj, zj, yj + zj, yj, yj - zj,
-                                        -xj - yj, -xj - zj, -xj + yj, -xj + zj, -xj, -zj, -yj - zj, -yj, -yj + zj,
-                                        zero};
-                const real_t sq[19] = {xj - yj, xj - zj, xj + yj, xj + zj, xj, zj, yj + zj, yj, yj - zj,
-                                       xj + yj, xj + zj, xj - yj, xj - zj, xj, zj, yj + zj, yj, yj - zj, zero};
-                const size_t id = (size_t)i + (size_t)nx * j + nxy * k;
-                for (int d = 0; d < 19; ++d) {
-                    const real_t c = rf * lin[d] + qf * (tre * sq[d] * sq[d] - cvsq);
-                    h[d][id] = crho * w[d] * (cte1 + c);
-                }
-            }
-        }
-    }
-    return 0;
+    return bgk::init_feq_host<real_t>(h, l, m, n, nx, ny, mode, u00);
 }
 
 // ---- device memory plumbing for hosts that have no other HIP runtime ----
@@ -393,6 +379,49 @@ int bgk_memcpy_d2h(void* dst, const void* src, size_t bytes)
 int bgk_device_count(int* n)
 {
     BGK_HIP_CHECK(hipGetDeviceCount(n));
+    return 0;
+}
+
+// ---- measured stream bandwidth: copy of n floats (n*8 bytes moved) -------
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ void stream_copy_kernel(const v4f* __restrict__ src, v4f* __restrict__ dst, long n4)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+int bgk_stream_copy(const void* src, void* dst, size_t n_floats)
+{
+    if (n_floats % 4) return -1;
+    const long n4 = (long)(n_floats / 4);
+    int blocks = (int)std::min<long>((n4 + 255) / 256, 256L * 8 * 8);
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, abi_stream(),
+                       reinterpret_cast<const v4f*>(src), reinterpret_cast<v4f*>(dst), n4);
+    report_launch("stream_copy_kernel");
+    return 0;
+}
+
+// ---- timing on the ABI stream (events bracket work queued there) --------
+int bgk_event_create(void** ev)
+{
+    hipEvent_t e;
+    BGK_HIP_CHECK(hipEventCreate(&e));
+    *ev = e;
+    return 0;
+}
+int bgk_event_record(void* ev)
+{
+    BGK_HIP_CHECK(hipEventRecord(reinterpret_cast<hipEvent_t>(ev), abi_stream()));
+    return 0;
+}
+int bgk_event_elapsed_ms(void* start, void* stop, float* ms)
+{
+    BGK_HIP_CHECK(hipEventSynchronize(reinterpret_cast<hipEvent_t>(stop)));
+    BGK_HIP_CHECK(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start), reinterpret_cast<hipEvent_t>(stop)));
+    return 0;
+}
+int bgk_event_destroy(void* ev)
+{
+    BGK_HIP_CHECK(hipEventDestroy(reinterpret_cast<hipEvent_t>(ev)));
     return 0;
 }
 

@@ -4,16 +4,10 @@
 // (SRC/alloca.F90:153-191): 19 (plus 18) separate column-major arrays of
 // shape (0:l+1+ipad, 0:m+1+jpad, 0:n+1), element (i,j,k) at i + nx*j + nxy*k.
 //
-// Design (not a translation of the reference's one-thread-per-cell 1-D
-// launch with 64-bit div/mod):
-//  * collide: wave64 rows along i (blockDim 64 x 4), a 3-D grid so no
-//    division is needed for the cell index, all 19 populations pulled in one
-//    pass with coalesced loads, results written with non-temporal stores
-//    (the output lattice is not read again in this step);
-//  * every population element is read exactly once per step (pull streaming
-//    is a permutation), so HBM traffic is 19 loads + 19 stores per cell;
-//  * boundary kernels cover only the surface and keep the reference's face
-//    order, since edge cells rely on the last writer.
+// The fused collide lives in collide.hpp.  The boundary kernels here fill
+// the halo exactly as the reference's routines do: they cover only the
+// surface and keep the reference's face order, since edge cells rely on the
+// last writer.
 #pragma once
 
 #include "common.hpp"
@@ -35,40 +29,6 @@ struct Geometry {
 __device__ __forceinline__ long gidx(const Geometry& g, int i, int j, int k)
 {
     return (long)i + (long)g.nx * (long)j + g.nxy * (long)k;
-}
-
-// -------------------------------------------------------------------------
-// Fused pull-stream + BGK collide, one step (SRC/col_MC.F90:126-536).
-// Reads a[0..18], writes b[0..17] and a[18] in place.
-// -------------------------------------------------------------------------
-constexpr int kColBX = 64;
-constexpr int kColBY = 4;
-
-template <typename R, int FMODE>
-__global__ __launch_bounds__(kColBX* kColBY) void col_mc_kernel(Lattice19<R> a, Lattice19<R> b, Geometry g,
-                                                               R omega, R fgrad, R ux, R uy, R uz,
-                                                               CollideConsts<R> cst)
-{
-    const int i = 1 + blockIdx.x * kColBX + threadIdx.x;
-    const int j = 1 + blockIdx.y * kColBY + threadIdx.y;
-    const int k = 1 + blockIdx.z;
-    if (i > g.l || j > g.m) return;
-    const long s = gidx(g, i, j, k);
-    const long nx = g.nx, nxy = g.nxy;
-
-    R x[19];
-#pragma unroll
-    for (int d = 0; d < 19; ++d) {
-        const long off = (long)cx_of(d) + nx * (long)cy_of(d) + nxy * (long)cz_of(d);
-        x[d] = __builtin_nontemporal_load(a.p[d] + (s - off));
-    }
-    R fx, fy, fz;
-    force_terms<R, FMODE>(density_ref(x, cst.cte1), fgrad, ux, uy, uz, fx, fy, fz);
-    R o[19];
-    bgk_collide_ref(x, omega, fx, fy, fz, cst, o);
-#pragma unroll
-    for (int d = 0; d < 18; ++d) __builtin_nontemporal_store(o[d], b.p[d] + s);
-    __builtin_nontemporal_store(o[18], a.p[18] + s);
 }
 
 // -------------------------------------------------------------------------

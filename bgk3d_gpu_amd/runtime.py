@@ -67,6 +67,11 @@ def v1(prec: str = "sp") -> ctypes.CDLL:
         lib.bgk_memcpy_h2d.argtypes = [vp, vp, ctypes.c_size_t]
         lib.bgk_memcpy_d2h.argtypes = [vp, vp, ctypes.c_size_t]
         lib.bgk_device_count.argtypes = [ctypes.POINTER(ci)]
+        lib.bgk_event_create.argtypes = [ctypes.POINTER(vp)]
+        lib.bgk_event_record.argtypes = [vp]
+        lib.bgk_event_elapsed_ms.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_float)]
+        lib.bgk_event_destroy.argtypes = [vp]
+        lib.bgk_stream_copy.argtypes = [vp, vp, ctypes.c_size_t]
         lib._bgk_typed = True
     return lib
 
@@ -110,5 +115,29 @@ class DeviceBuffer:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    """hipEvent recorded on the library's ABI stream."""
+
+    def __init__(self, lib):
+        self.lib = lib
+        p = ctypes.c_void_p()
+        check(lib.bgk_event_create(ctypes.byref(p)), "hipEventCreate")
+        self.ptr = p.value
+
+    def record(self):
+        check(self.lib.bgk_event_record(self.ptr), "hipEventRecord")
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = ctypes.c_float()
+        check(self.lib.bgk_event_elapsed_ms(self.ptr, later.ptr, ctypes.byref(ms)), "hipEventElapsedTime")
+        return ms.value
+
+    def __del__(self):
+        try:
+            self.lib.bgk_event_destroy(self.ptr)
         except Exception:
             pass

@@ -1,0 +1,103 @@
+/*
+ * bgk3d_engine.h -- handle API of the gfx950 D3Q19 BGK step (v2).
+ *
+ * libbgk3d_engine.so owns the lattices, so it can lay them out for the
+ * hardware (rows padded so the first interior cell of every row starts a
+ * 128-byte line), fold every boundary condition into the pull (no halo
+ * kernels per step), store populations in FP32, FP64, FP16 or BF16 (shifted
+ * f - w, collision in FP32 or FP64), and batch many small solves into one
+ * persistent kernel (bgks_*).
+ *
+ * The reference has no such API: its driver owns the arrays and calls the
+ * per-routine symbols of bgk3d_gpu.h (SRC/bgk3d.F90:169-196).  Each field
+ * below names the reference input it stands for.  All functions return 0 on
+ * success and a negative BGK_E* code on error (bgk_last_error() gives text).
+ */
+#ifndef BGK3D_ENGINE_H
+#define BGK3D_ENGINE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum bgk_boundary {
+    BGK_BC_DRIVEN = 0,   /* lid-driven cavity, SRC/bcond.driven.F90 (LDC build) */
+    BGK_BC_PERIODIC = 1, /* fully periodic, SRC/bcond.periodic.F90 (TGV/HIT builds) */
+    BGK_BC_CHANNEL = 2,  /* x,y periodic, walls z=1,n, body force, SRC/bcond.channel.F90 (POF) */
+    BGK_BC_INFLOW = 3    /* inflow x=0, outflow x=l+1, y,z periodic, SRC/bcond.inflow.F90 (VKS) */
+};
+
+enum bgk_storage { BGK_F32 = 0, BGK_F64 = 1, BGK_F16 = 2, BGK_BF16 = 3 };
+
+enum bgk_error {
+    BGK_OK = 0,
+    BGK_EINVAL = -1,   /* bad argument */
+    BGK_ENOMEM = -2,   /* device allocation failed */
+    BGK_EHIP = -3,     /* HIP runtime error */
+    BGK_EBUSY = -4     /* serving queue full / slot unavailable */
+};
+
+typedef struct bgk_config {
+    int l, m, n;          /* interior size (namelist lx, ly, lz; SRC/input.F90:44-48) */
+    int boundary;         /* enum bgk_boundary (compile-time case flag in the reference) */
+    int storage;          /* enum bgk_storage (SINGLE / DOUBLE_P; F16/BF16 are new) */
+    int rounding;         /* 0 nearest-even, 1 stochastic (F16/BF16 stores only) */
+    double svisc;         /* viscosity nu (namelist svisc); omega = 2/(6 nu + 1) */
+    double u0;            /* lid velocity, or volume-force scale for the channel (namelist u0) */
+    int init;             /* 0 rest, 1 3-D Taylor-Green (SRC/init.F90) */
+    int obstacle;         /* 0 none, 1 hollow sphere shell (SRC/build.obs.F90), 2 solid sphere */
+    int obs_ic, obs_jc, obs_kc, obs_radius; /* sphere centre/radius; < 0 = reference defaults */
+    double u_inflow;      /* inflow velocity; < 0 = reference value (default-real 0.1) */
+    int force_axis;       /* channel body-force axis 1|2|3 (FORCING_Y/Z select 2/3) */
+    int device;           /* HIP device ordinal */
+    unsigned seed;        /* stochastic-rounding stream */
+} bgk_config;
+
+typedef struct bgk_handle bgk_handle;
+
+/* Fill `cfg` with the reference's defaults for a driven cavity of l x m x n. */
+void bgk_config_default(bgk_config* cfg, int l, int m, int n);
+
+int bgk_create(const bgk_config* cfg, bgk_handle** out);
+int bgk_destroy(bgk_handle* h);
+
+/* Queue nsteps time steps (boundaries folded in + fused collide + swap) on
+ * the handle's stream; returns without waiting. */
+int bgk_step(bgk_handle* h, int nsteps);
+int bgk_sync(bgk_handle* h);
+
+/* Run nsteps and time them: total wall time of the steps (hipEvents on the
+ * handle's stream) and the mean duration of the fused collide kernel
+ * (events bracketing each launch). */
+int bgk_time_steps(bgk_handle* h, int nsteps, double* total_ms, double* collide_avg_ms);
+
+/* Global means {<u>, <v>, <w>, <rho>, <0.5|u|^2>} (SRC/diagno.F90), FP64 sums. */
+int bgk_diagno(bgk_handle* h, double out[5]);
+
+/* Momentum-exchange force on the obstacle / (u_inflow^2 radius)
+ * (SRC/draglift.F90), FP64 sums; needs an obstacle. */
+int bgk_draglift(bgk_handle* h, double out[3]);
+
+/* Interior macroscopic fields, i fastest then j then k (l*m*n values each);
+ * any pointer may be NULL. */
+int bgk_fields(bgk_handle* h, double* rho, double* ux, double* uy, double* uz);
+
+/* Interior values of population d (0..18, direction d+1), as double. */
+int bgk_get_population(bgk_handle* h, int d, double* out);
+/* Overwrite interior values of population d (rounded to the storage type). */
+int bgk_set_population(bgk_handle* h, int d, const double* in);
+
+/* Sizes: [0] nx, [1] ny, [2] nz, [3] bytes of device memory held,
+ * [4] storage bytes per population value, [5] steps taken. */
+int bgk_info(bgk_handle* h, long long out[6]);
+
+/* Text of the last error on this thread. */
+const char* bgk_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BGK3D_ENGINE_H */

@@ -152,6 +152,16 @@ int bgk_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int bgk_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int bgk_device_count(int* n);
 
+/* Plain device copy of n_floats floats (n_floats % 4 == 0) on the ABI
+ * stream: the measured-stream-bandwidth probe (8 bytes moved per float). */
+int bgk_stream_copy(const void* src, void* dst, size_t n_floats);
+
+/* Event timing on the stream selected by bgk_set_stream (hipEvent_t handles). */
+int bgk_event_create(void** ev);
+int bgk_event_record(void* ev);
+int bgk_event_elapsed_ms(void* start, void* stop, float* ms);
+int bgk_event_destroy(void* ev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,8 @@
 ! MODE 0: full loop (boundaries -> collision) for ITFIN steps, with the
 !         reference's diagno/prof_k (and draglift under DRAG) every ICHECK;
 ! MODE 1: one call of `boundaries` only;
-! MODE 2: one call of `col_MC` only (includes the a<->b pointer swap).
+! MODE 2: one call of `col_MC` only (includes the a<->b pointer swap);
+! MODE 3: like MODE 0 but timing only: no diagnostics, no output files.
 ! INIT_V >= 0 calls the reference's init(INIT_V); INIT_V = -1 reads the
 ! 19 populations (full halo arrays, storage kind, raw) from OUTDIR/in.bin.
 ! Output: OUTDIR/out.bin (a01..a19 after the run, raw), the reference's
@@ -81,14 +82,16 @@
       else if (mode == 2) then
          call col_MC(1)
       else
-         call diagno(0)
-         call varm(0)
-         call prof_k(0, l/2, m/2)
+         if (mode == 0) then
+            call diagno(0)
+            call varm(0)
+            call prof_k(0, l/2, m/2)
+         endif
          call system_clock(c0, crate, cmax)
          do itime = 1, itfin
             call boundaries
             call collision(itime)
-            if (mod(itime, icheck) == 0) then
+            if (mode == 0 .and. mod(itime, icheck) == 0) then
                call diagno(itime)
                call prof_k(itime, l/2, m/2)
 #ifdef DRAG
@@ -98,7 +101,7 @@
          enddo
          call system_clock(c1, crate, cmax)
          tloop = real(c1-c0,8)/real(crate,8)
-         call varm(itfin)
+         if (mode == 0) call varm(itfin)
          if (tloop > 0.0d0) then
             write(6,'(a,es16.8)') '# Mlups ', &
                  real(l,8)*real(m,8)*real(n,8)*real(itfin,8)/tloop/1.0d6
@@ -106,6 +109,7 @@
          write(6,'(a,es16.8)') '# seconds ', tloop
       endif
 
+      if (mode == 3) stop
       open(99, file=trim(outdir)//'/out.bin', access='stream', &
            form='unformatted', status='replace')
       write(99) a01,a02,a03,a04,a05,a06,a07,a08,a09,a10, &

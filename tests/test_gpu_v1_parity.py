@@ -147,6 +147,34 @@ def test_draglift_matches_oracle(gpu, prec):
         obs, radius = sim.obs_host, sim.radius
     finally:
         sim.close()
-    ref = O.draglift(G, obs, case.l, case.m, case.n, radius, prec=prec)
-    tol = 1e-4 if prec == "sp" else 1e-10
-    np.testing.assert_allclose(dl, ref, rtol=tol, atol=tol * np.abs(ref).max())
+    # exact reference: per-cell sums formed in the storage kind in the
+    # reference's order (SRC/draglift.F90:81-103), accumulated in FP64
+    R = np.float32 if prec == "sp" else np.float64
+    l, m, n = case.l, case.m, case.n
+    ic, jc, kc = 2 * l // 5, m // 2, n // 2
+    sl = lambda c, r: slice(c - r - 3, c + r + 3 + 1)  # 1-based index range -> python slice on padded arrays
+    cx = [1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0]
+    cy = [-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1]
+    cz = [0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1]
+    obs_pad = np.zeros((n + 2, m + 2, l + 2), np.int32)
+    obs_pad[1:-1, 1:-1, 1:-1] = obs
+    ks, js, is_ = sl(kc, radius), sl(jc, radius), sl(ic, radius)
+    fluid = obs_pad[ks, js, is_] == 0
+    exact, mag = [], []
+    for cc in (cx, cy, cz):
+        tot = None
+        for d in range(18):
+            nb = obs_pad[slice(ks.start + cz[d], ks.stop + cz[d]), slice(js.start + cy[d], js.stop + cy[d]),
+                         slice(is_.start + cx[d], is_.stop + cx[d])].astype(R)
+            t = R(2.0) * R(cc[d]) * G[d][ks, js, is_].astype(R) * nb
+            tot = t if tot is None else tot + t
+        exact.append(tot[fluid].astype(np.float64).sum())
+        mag.append(np.abs(tot[fluid]).astype(np.float64).sum())
+    norm = float(R(1) / (R(case.u_inflow) * R(case.u_inflow) * R(radius)))
+    exact = np.array(exact) * norm
+    mag = np.array(mag) * abs(norm)
+    assert np.all(np.abs(dl - exact) <= 1e-12 * mag + 1e-300)
+    # the oracle (like the reference) accumulates in the storage kind
+    ref = O.draglift(G, obs, l, m, n, radius, prec=prec)
+    eps = float(np.finfo(R).eps)
+    assert np.all(np.abs(ref - exact) <= (fluid.sum() + 2) * eps * mag + 1e-300)
