@@ -65,12 +65,44 @@ def build_one(name: str, sources, extra, verbose=False) -> Path:
     return out
 
 
+def build_fortran(verbose=False):
+    """Fortran interface module + host program (amdflang), linked against
+    libbgk3d_gpu_{sp,dp}.so with an $ORIGIN-relative rpath."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not Path(fc).exists():
+        return []
+    src = PKG / "fortran"
+    out = src / "bin"
+    out.mkdir(exist_ok=True)
+    built = []
+    import tempfile
+    for prec, define in (("sp", []), ("dp", ["-DDOUBLE_P"])):
+        with tempfile.TemporaryDirectory() as tmp:
+            steps = [
+                [fc, "-cpp", *define, "-O2", "-module-dir", tmp, "-c", str(src / "bgk3d_gpu_mod.F90"), "-o",
+                 f"{tmp}/mod.o"],
+                [fc, "-cpp", *define, "-O2", "-module-dir", tmp, f"-I{tmp}", "-c", str(src / "bgk3d_host.F90"), "-o",
+                 f"{tmp}/host.o"],
+                [fc, f"{tmp}/mod.o", f"{tmp}/host.o", f"-L{LIBDIR}", f"-lbgk3d_gpu_{prec}",
+                 "-Wl,-rpath,$ORIGIN/../../lib", "-o", str(out / f"bgk3d_host_{prec}.x")],
+            ]
+            for cmd in steps:
+                if verbose:
+                    print(" ".join(cmd), flush=True)
+                r = subprocess.run(cmd, capture_output=True, text=True)
+                if r.returncode != 0:
+                    raise RuntimeError(f"Fortran host build failed:\n{r.stderr}")
+        built.append(out / f"bgk3d_host_{prec}.x")
+    return built
+
+
 def build_all(verbose=False):
     from concurrent.futures import ThreadPoolExecutor
 
     with ThreadPoolExecutor(max_workers=len(TARGETS)) as ex:
         futs = [ex.submit(build_one, n, s, e, verbose) for n, (s, e) in TARGETS.items()]
-        return [f.result() for f in futs]
+        libs = [f.result() for f in futs]
+    return libs + build_fortran(verbose)
 
 
 if __name__ == "__main__":

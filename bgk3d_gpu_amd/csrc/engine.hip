@@ -697,6 +697,32 @@ int bgk_fields(bgk_handle* h, double* rho, double* ux, double* uy, double* uz)
     return rc;
 }
 
+int bgk_prof_k(bgk_handle* h, int icoord, int jcoord, double* out)
+{
+    if (!h || !out) return fail(BGK_EINVAL, "bad arguments");
+    const bgk_config& c = h->cfg;
+    if (icoord < 1 || icoord > c.l || jcoord < 1 || jcoord > c.m)
+        return fail(BGK_EINVAL, "line (%d,%d) outside the interior", icoord, jcoord);
+    ENG_HIP(hipSetDevice(c.device));
+    return by_storage(c.storage, [&](auto tag) {
+        typedef decltype(tag) S;
+        typedef typename Storage<S>::compute C;
+        C* tmp = nullptr;
+        if (hipMalloc(&tmp, (size_t)4 * c.n * sizeof(C)) != hipSuccess) return fail(BGK_ENOMEM, "prof_k");
+        hipLaunchKernelGGL(prof_k_kernel<S>, dim3(ceil_div(c.n, 64)), dim3(64), 0, h->stream, h->current<S>(),
+                           h->geom(), icoord, jcoord, cte_one<S>(), tmp);
+        std::vector<C> host((size_t)4 * c.n);
+        int r = 0;
+        if (hipMemcpyAsync(host.data(), tmp, host.size() * sizeof(C), hipMemcpyDeviceToHost, h->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess)
+            r = fail(BGK_EHIP, "prof_k download");
+        for (size_t q = 0; q < host.size(); ++q) out[q] = (double)host[q];
+        hipFree(tmp);
+        return r;
+    });
+}
+
 int bgk_get_population(bgk_handle* h, int dir, double* out)
 {
     if (!h || !out || dir < 0 || dir > 18) return fail(BGK_EINVAL, "bad arguments");

@@ -42,6 +42,7 @@ def lib() -> ctypes.CDLL:
         L.bgk_draglift.argtypes = [vp, vp]
         L.bgk_fields.argtypes = [vp, vp, vp, vp, vp]
         L.bgk_get_population.argtypes = [vp, ci, vp]
+        L.bgk_prof_k.argtypes = [vp, ci, ci, vp]
         L.bgk_set_population.argtypes = [vp, ci, vp]
         L.bgk_info.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
         L.bgk_last_error.restype = ctypes.c_char_p
@@ -112,6 +113,14 @@ class Engine:
         c = self.case
         out = [np.empty((c.n, c.m, c.l)) for _ in range(4)]
         _check(self.L.bgk_fields(self.h, *[o.ctypes.data for o in out]), "bgk_fields")
+        return out
+
+    def prof_k(self, ic=None, jc=None) -> np.ndarray:
+        """(n, 4) array of u, w (y-velocity), v (z-velocity), rho along k."""
+        c = self.case
+        out = np.zeros((c.n, 4))
+        _check(self.L.bgk_prof_k(self.h, c.l // 2 if ic is None else ic, c.m // 2 if jc is None else jc,
+                                 out.ctypes.data), "bgk_prof_k")
         return out
 
     def population(self, d: int) -> np.ndarray:

@@ -84,6 +84,11 @@ int bgk_draglift(bgk_handle* h, double out[3]);
  * any pointer may be NULL. */
 int bgk_fields(bgk_handle* h, double* rho, double* ux, double* uy, double* uz);
 
+/* Velocity/density line along k at (icoord, jcoord), as the reference's
+ * prof_k prints it (SRC/prof_k.F90:47-79): out[4*(k-1)+{0,1,2,3}] =
+ * {u, w (y-velocity), v (z-velocity), rho}, computed in the collision kind. */
+int bgk_prof_k(bgk_handle* h, int icoord, int jcoord, double* out);
+
 /* Interior values of population d (0..18, direction d+1), as double. */
 int bgk_get_population(bgk_handle* h, int d, double* out);
 /* Overwrite interior values of population d (rounded to the storage type). */

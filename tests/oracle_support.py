@@ -246,3 +246,52 @@ def run_ref(case, l, m, n, steps, prec="sp", svisc=0.064, u0=0.1, icheck=None, i
     finally:
         if keep_dir is None:
             shutil.rmtree(d, ignore_errors=True)
+
+
+# ---------------------------------------------------------------------------
+# Fortran-style formatting, to compare against the reference's text outputs
+# ---------------------------------------------------------------------------
+
+def fortran_e(x: float, w: int = 14, d: int = 6) -> str:
+    """Fortran Ew.d edit descriptor: 0.dddddd E+xx, right-justified."""
+    x = float(x)
+    if x == 0.0:
+        body = "0." + "0" * d + "E+00"
+        return body.rjust(w)
+    mant, exp = f"{abs(x):.{d - 1}e}".split("e")
+    digits = mant.replace(".", "")
+    e = int(exp) + 1
+    body = f"{'-' if x < 0 else ''}0.{digits}E{'+' if e >= 0 else '-'}{abs(e):02d}"
+    return body.rjust(w)
+
+
+def diagno_line(itime: int, vals) -> str:
+    """SRC/diagno.F90:145,159: (i8,5(e14.6,1x)) of itime, <u>, <v>, <w>, <rho>, <ke>."""
+    return f"{itime:8d}" + "".join(fortran_e(v) + " " for v in vals)
+
+
+def parse_blocks(text: str):
+    """Split a prof_k.dat / u_med.dat text into {t: [[floats], ...]} blocks."""
+    blocks, cur, t = {}, None, None
+    for line in text.splitlines():
+        if line.startswith("# t="):
+            t = int(line.split("=")[1])
+            cur = blocks.setdefault(t, [])
+        elif line.strip() and cur is not None:
+            cur.append(line)
+    return blocks
+
+
+CX = [1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0]
+CY = [-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1, 0]
+CZ = [0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1, 0]
+
+
+def pulled_set(A, l, m, n):
+    """Concatenation, over d, of the cells the next col_MC pull reads from
+    population d: x - c_d for every interior x (SRC/col_MC.F90:275-293)."""
+    parts = []
+    for d in range(19):
+        parts.append(np.ascontiguousarray(
+            A[d, 1 - CZ[d]:n + 1 - CZ[d], 1 - CY[d]:m + 1 - CY[d], 1 - CX[d]:l + 1 - CX[d]]).ravel())
+    return np.concatenate(parts)

@@ -1,0 +1,126 @@
+"""CPU checks of the C-ABI libraries and the host-side logic (no GPU calls).
+
+* every function declared in include/*.h is exported by the library that
+  implements it;
+* host-side pieces restated in the product (initial state, obstacle mask,
+  hencol parameters, namelist reader) agree with the oracle / reference data;
+* without a GPU the product fails loudly instead of falling back to the CPU.
+"""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_support as O
+from bgk3d_gpu_amd import runtime as rt
+from bgk3d_gpu_amd.cases import Case, case_from_namelist, read_namelist, sphere_obstacle
+
+REPO = Path(__file__).resolve().parents[1]
+GOLD = Path(__file__).parent / "golden" / "reference_data"
+
+
+def declared(header):
+    text = (REPO / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:void|int|char|double|float)\s*\**\s*(\w+)\s*\(", text,
+                                 re.M)))
+
+
+def exported(lib):
+    return {name for name in dir(lib)}
+
+
+@pytest.mark.parametrize("libname,header", [("libbgk3d_gpu_sp.so", "bgk3d_gpu.h"),
+                                            ("libbgk3d_gpu_dp.so", "bgk3d_gpu.h"),
+                                            ("libbgk3d_engine.so", "bgk3d_engine.h")])
+def test_library_exports_every_declared_symbol(libname, header):
+    lib = ctypes.CDLL(str(rt.LIBDIR / libname))
+    names = declared(header)
+    assert len(names) >= 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"{libname} lacks {missing}"
+
+
+def test_reference_symbols_present():
+    # exactly the names the reference's Fortran binds (SRC/col_MC.F90:101-123,
+    # SRC/bcond_gpu_mod.F90:30-77)
+    names = declared("bgk3d_gpu.h")
+    for s in ("col_mc_gpu", "bcond_periodic_gpu", "bcond_channel_gpu", "bcond_driven_gpu", "bcond_inflow_gpu",
+              "bcond_obs_gpu", "gpu_device_sync"):
+        assert s in names
+
+
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+@pytest.mark.parametrize("mode,u00", [(0, 0.1), (1, 0.05)])
+def test_host_init_matches_oracle(prec, mode, u00):
+    lib = rt.v1(prec)
+    R = rt.real_ctype(prec)
+    l, m, n = 21, 17, 13
+    A = O.new_lattice(l, m, n, prec, ipad=5, jpad=2)
+    O.init(A, l, m, n, prec, mode, u00)
+    H = O.new_lattice(l, m, n, prec, ipad=5, jpad=2)
+    P = (ctypes.c_void_p * 19)(*[H[d].ctypes.data for d in range(19)])
+    lib.bgk_init_host.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [R]
+    assert lib.bgk_init_host(P, l, m, n, H.shape[3], H.shape[2], mode, R(u00)) == 0
+    assert np.array_equal(A.view(np.uint8), H.view(np.uint8))
+
+
+@pytest.mark.parametrize("lmn", [(40, 24, 24), (100, 48, 48), (64, 300, 300)])
+def test_sphere_obstacle_matches_build_obs(lmn):
+    l, m, n = lmn
+    obs, box, radius, centre = sphere_obstacle(l, m, n)
+    oobs, obox, orad = O.build_obs(l, m, n)
+    assert radius == orad and box == obox
+    assert np.array_equal(obs, oobs)
+    assert centre == (2 * l // 5, m // 2, n // 2)
+
+
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+def test_case_hencol_matches_oracle(prec):
+    for svisc, u0, m in ((0.064, 0.1, 64), (0.1, 0.02048, 128), (0.0256, 0.1, 256), (0.01, 0.05, 256)):
+        case = Case("channel", 8, m, 8, svisc=svisc, u0=u0)
+        om, fg = case.hencol(prec)
+        oom, ofg = O.hencol(svisc, u0, m, prec)
+        assert om == oom and fg == ofg
+        case = Case("ldc", 8, m, 8, svisc=svisc, u0=u0)
+        assert case.hencol(prec)[1] == 0.0  # no volume force outside the channel build
+
+
+def test_namelists_of_reference_tests():
+    nml = read_namelist(GOLD / "driven_cavity_256_re1000_bgk.input")
+    assert (nml["lx"], nml["ly"], nml["lz"]) == (256, 256, 256)
+    assert nml["svisc"] == pytest.approx(0.0256) and nml["u0"] == pytest.approx(0.1)
+    assert nml["itfin"] == 100000 and nml["icheck"] == 20000
+    case, ctl = case_from_namelist(GOLD / "driven_cavity_256_re1000_bgk.input", "ldc")
+    assert case.l == 256 and case.init == 0 and ctl["icheck"] == 20000
+    case, _ = case_from_namelist(GOLD / "Sphere_bgk.input", "inflow")
+    assert (case.l, case.m, case.n) == (900, 150, 150) and case.obstacle
+    for f in GOLD.glob("*_bgk.input"):
+        read_namelist(f)
+
+
+def test_reference_profile_fixture_is_whole():
+    blocks = O.parse_blocks((GOLD / "driven_cavity_256_re1000_prof_k.dat").read_text())
+    assert sorted(blocks) == [0, 20000, 40000, 60000, 80000, 100000]
+    assert all(len(rows) == 256 for rows in blocks.values())
+    last = np.array([[float(v) for v in r.split()] for r in blocks[100000]])
+    assert last[-1, 1] == pytest.approx(0.0963182, rel=1e-6)
+
+
+def test_case_validation():
+    with pytest.raises(ValueError):
+        Case("nope")
+    with pytest.raises(ValueError):
+        Case("ldc", 0, 4, 4)
+
+
+def test_no_cpu_fallback_without_gpu():
+    from conftest import gpu_present
+    if gpu_present():
+        pytest.skip("a GPU is present")
+    from bgk3d_gpu_amd.engine import Engine
+    with pytest.raises(rt.BGKError):
+        Engine(Case("ldc", 8, 8, 8), "f32")
+    assert rt.device_count() == 0
