@@ -41,6 +41,12 @@ def test_fortran_host_reproduces_reference(gpu, tmp_path, key):
     A = np.fromfile(tmp_path / "out.bin", dtype=dt).reshape(19, n + 2, m + 2, l + 2)
     I = np.ascontiguousarray(A[:, 1:-1, 1:-1, 1:-1])
     assert hashlib.sha256(I.tobytes()).hexdigest() == spec["sha256_interior"]
-    ref_lines = [x.rstrip() for x in spec["texts"]["diagno.dat"].splitlines() if x.strip()][1:]  # skip t=0
-    ours = [x.rstrip() for x in (tmp_path / "diagno.dat").read_text().splitlines() if x.strip()]
-    assert ours == ref_lines
+    # diagno.dat: same steps; values equal to print precision.  The means are
+    # FP64 sums over cells (the device sums in a different order), so a mean
+    # that is zero by symmetry (~1e-18 of rounding noise) may differ in its
+    # digits; compare with an absolute floor.
+    ref_rows = [[float(v) for v in x.split()] for x in spec["texts"]["diagno.dat"].splitlines() if x.strip()][1:]
+    our_rows = [[float(v) for v in x.split()] for x in (tmp_path / "diagno.dat").read_text().splitlines()
+                if x.strip()]
+    assert [r[0] for r in our_rows] == [r[0] for r in ref_rows]
+    np.testing.assert_allclose(np.array(our_rows)[:, 1:], np.array(ref_rows)[:, 1:], rtol=1e-5, atol=1e-15)
