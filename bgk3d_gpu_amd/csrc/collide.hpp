@@ -56,7 +56,6 @@ struct FusedArgs {
     uint32_t seed;   // stochastic rounding stream (step-dependent)
 };
 
-constexpr int kWaveRows = 4;  // rows (waves) per 256-thread block
 
 template <int AX>
 __device__ __forceinline__ void axis_fix(int cd, int s, int lo_hi_n, int& shift, bool& bounce, bool& top)
@@ -118,11 +117,52 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
     }
 }
 
+// Work mapping.  wpr = waves needed per (possibly shifted) row.  A block
+// holds whole rows whenever it can, so the cache line two neighbouring
+// 64-cell segments share for the +-x pulls is fetched once, by one CU:
+//   4 <= wpr <= 8   one row per block, wpr waves;
+//   wpr > 8         4 consecutive segments of one row per block;
+//   wpr < 4         4/wpr rows per block.
+struct RowMap {
+    int wpr;   // waves per row
+    int wpb;   // waves per block
+    int rpb;   // rows per block
+    int bpr;   // blocks per row
+};
+
+inline RowMap row_map(int span)
+{
+    RowMap r;
+    r.wpr = ceil_div(span, 64);
+    if (r.wpr > 8) {
+        r.wpb = 4;
+        r.rpb = 1;
+        r.bpr = ceil_div(r.wpr, 4);
+    } else if (r.wpr >= 4) {
+        r.wpb = r.wpr;
+        r.rpb = 1;
+        r.bpr = 1;
+    } else {
+        r.rpb = 4 / r.wpr;
+        r.wpb = r.rpb * r.wpr;
+        r.bpr = 1;
+    }
+    return r;
+}
+
 template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
-__global__ __launch_bounds__(64 * kWaveRows) void collide_fused(FusedArgs<S> p)
+__global__ __launch_bounds__(512) void collide_fused(FusedArgs<S> p, RowMap rm)
 {
     typedef typename Storage<S>::compute C;
-    const int j = 1 + blockIdx.y * kWaveRows + threadIdx.y;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int seg, j;
+    if (rm.rpb == 1) {
+        seg = (int)blockIdx.x * rm.wpb + wave;
+        j = 1 + (int)blockIdx.y;
+    } else {
+        seg = wave % rm.wpr;
+        j = 1 + (int)blockIdx.y * rm.rpb + wave / rm.wpr;
+    }
     const int k = 1 + blockIdx.z;
     if (j > p.m) return;
     int shift = 0;
@@ -131,7 +171,7 @@ __global__ __launch_bounds__(64 * kWaveRows) void collide_fused(FusedArgs<S> p)
                            p.nxy * (long)k;
         shift = (int)(first & (long)(p.align - 1));
     }
-    const int i = 1 - shift + (int)blockIdx.x * 64 + (int)threadIdx.x;
+    const int i = 1 - shift + seg * 64 + lane;
     if (i < 1 || i > p.l) return;
     const long s = (long)i + (long)p.nx * j + p.nxy * (long)k;
 
@@ -145,7 +185,10 @@ __global__ __launch_bounds__(64 * kWaveRows) void collide_fused(FusedArgs<S> p)
     C fx, fy, fz;
     force_terms<C, FMODE>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
     C o[19];
-    bgk_collide_ref(x, p.omega, fx, fy, fz, p.cst, o);
+    if constexpr (sizeof(S) == 2)
+        bgk_collide_fast(x, p.omega, fx, fy, fz, p.cst, o);
+    else
+        bgk_collide_ref(x, p.omega, fx, fy, fz, p.cst, o);
 
     uint32_t r = 0;
     if (SR) r = hash32(p.seed ^ hash32((uint32_t)s));
@@ -157,12 +200,12 @@ __global__ __launch_bounds__(64 * kWaveRows) void collide_fused(FusedArgs<S> p)
     }
 }
 
-// Launch geometry shared by both ABIs.
-inline dim3 fused_grid(int l, int m, int n, int align)
+// Launch geometry shared by both ABIs; `span` is the row length the waves
+// must cover (l, or l plus the largest alignment shift).
+inline dim3 fused_grid(const RowMap& rm, int m, int n)
 {
-    const int span = align ? l + align - 1 : l;  // worst-case shifted row span
-    return dim3((unsigned)ceil_div(span, 64), (unsigned)ceil_div(m, kWaveRows), (unsigned)n);
+    return dim3((unsigned)rm.bpr, (unsigned)ceil_div(m, rm.rpb), (unsigned)n);
 }
-inline dim3 fused_block() { return dim3(64, kWaveRows); }
+inline dim3 fused_block(const RowMap& rm) { return dim3((unsigned)(64 * rm.wpb)); }
 
 }  // namespace bgk

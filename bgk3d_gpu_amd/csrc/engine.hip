@@ -333,8 +333,9 @@ void launch_fused(bgk_handle& h)
     p.onk = h.ob[5];
     p.align = 0;
     p.seed = hash32(h.cfg.seed ^ hash32((uint32_t)h.steps * 0x9e3779b9U + 1u));
-    hipLaunchKernelGGL((collide_fused<S, BX, BY, BZ, OB, FM, SR>), fused_grid(p.l, p.m, p.n, 0), fused_block(), 0,
-                       h.stream, p);
+    const RowMap rm = row_map(p.l);
+    hipLaunchKernelGGL((collide_fused<S, BX, BY, BZ, OB, FM, SR>), fused_grid(rm, p.m, p.n), fused_block(rm), 0,
+                       h.stream, p, rm);
 }
 
 template <typename S, int SR>

@@ -101,9 +101,9 @@ void launch_col(const Lattice19<real_t>& A, const Lattice19<real_t>& B, const Ge
         const long first = (long)(reinterpret_cast<uintptr_t>(p.b[0]) / sizeof(real_t)) + 1;
         span = g.l + (int)(first & (p.align - 1));
     }
-    dim3 grid((unsigned)ceil_div(span, 64), (unsigned)ceil_div(g.m, kWaveRows), (unsigned)g.n);
-    hipLaunchKernelGGL((collide_fused<real_t, AX_HALO, AX_HALO, AX_HALO, false, FM, 0>), grid, fused_block(), 0,
-                       abi_stream(), p);
+    const RowMap rm = row_map(span);
+    hipLaunchKernelGGL((collide_fused<real_t, AX_HALO, AX_HALO, AX_HALO, false, FM, 0>), fused_grid(rm, g.m, g.n),
+                       fused_block(rm), 0, abi_stream(), p, rm);
     report_launch("collide_fused");
 }
 
@@ -384,10 +384,12 @@ int bgk_device_count(int* n)
 
 // ---- measured stream bandwidth: copy of n floats (n*8 bytes moved) -------
 typedef float v4f __attribute__((ext_vector_type(4)));
+// plain loads and non-temporal stores, the same memory instructions as the
+// fused collide, so its counters calibrate the collide's
 __global__ void stream_copy_kernel(const v4f* __restrict__ src, v4f* __restrict__ dst, long n4)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x)
-        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+        __builtin_nontemporal_store(src[i], dst + i);
 }
 int bgk_stream_copy(const void* src, void* dst, size_t n_floats)
 {
