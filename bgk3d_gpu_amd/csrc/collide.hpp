@@ -190,12 +190,14 @@ __global__ __launch_bounds__(512) void collide_fused(FusedArgs<S> p, RowMap rm)
     else
         bgk_collide_ref(x, p.omega, fx, fy, fz, p.cst, o);
 
+    // stochastic rounding: one hash per cell and step, then one LCG step per
+    // store (deterministic: independent of scheduling)
     uint32_t r = 0;
-    if (SR) r = hash32(p.seed ^ hash32((uint32_t)s));
+    if (SR) r = hash32(p.seed ^ ((uint32_t)s * 0x9e3779b1u));
 #pragma unroll
     for (int d = 0; d < 19; ++d) {
         S* dst = (d < 18 ? p.b[d] : p.a[18]) + s;
-        if (SR) r = hash32(r + 0x9e3779b9U);
+        if (SR) r = lcg_next(r);
         store_nt(dst, Rounder<S, SR>::cvt(o[d], r));
     }
 }

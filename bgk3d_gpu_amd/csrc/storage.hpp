@@ -68,7 +68,9 @@ __host__ __device__ inline uint32_t hash32(uint32_t x)
 }
 
 // Round a float to the storage type.  SR = 0: nearest-even; SR = 1:
-// stochastic with the 32-bit random word r.
+// stochastic with the 32-bit random word r (only its high bits are used,
+// which is where the per-store LCG stream below is strongest).
+__device__ __forceinline__ uint32_t lcg_next(uint32_t r) { return r * 747796405u + 2891336453u; }
 template <typename S, int SR>
 struct Rounder;
 
@@ -86,26 +88,26 @@ struct Rounder<f16_t, 0> {
 };
 template <>
 struct Rounder<f16_t, 1> {
-    // unbiased stochastic rounding: take the nearest half h, then move to
-    // its neighbour on the side of the residual with probability
-    // |residual| / spacing, so that E[result] = v
+    // Unbiased stochastic rounding to FP16 from the FP32 bits: in the FP16
+    // normal range the 13 mantissa bits FP16 drops receive a uniform 13-bit
+    // addend and are truncated (a carry rounds up with probability equal to
+    // the dropped fraction); below it the spacing is the fixed 2^-24, so
+    // |v|*2^24 plus a uniform [0,1) is floored.  Both results are exact FP16
+    // values, so the final conversion does not round again.  Uses the top
+    // bits of r.
     __device__ __forceinline__ static f16_t cvt(float v, uint32_t r)
     {
-        const f16_t h = (f16_t)v;
-        const float hf = (float)h;
-        const float res = v - hf;
-        if (res == 0.0f || !(__builtin_fabsf(hf) < 65504.0f)) return h;
-        const uint16_t hb = __builtin_bit_cast(uint16_t, h);
-        const bool hneg = (hb & 0x8000u) != 0, hzero = (hb & 0x7fffu) == 0;
-        uint16_t nb;
-        if (res > 0.0f) {
-            nb = hzero ? (uint16_t)0x0001u : (hneg ? (uint16_t)(hb - 1u) : (uint16_t)(hb + 1u));
-        } else {
-            nb = hzero ? (uint16_t)0x8001u : (hneg ? (uint16_t)(hb + 1u) : (uint16_t)(hb - 1u));
+        const uint32_t u = __float_as_uint(v);
+        const uint32_t sign = u & 0x80000000u;
+        uint32_t a = u ^ sign;
+        if (a >= 0x38800000u) {                        // |v| >= 2^-14
+            if (a >= 0x477fe000u) return (f16_t)v;     // at the top of the range: nearest
+            a = (a + (r >> 19)) & ~0x1fffu;
+            return (f16_t)__uint_as_float(a | sign);
         }
-        const float gap = __builtin_fabsf((float)__builtin_bit_cast(f16_t, nb) - hf);
-        const float u = (float)(r >> 8) * (1.0f / 16777216.0f);
-        return (u * gap < __builtin_fabsf(res)) ? __builtin_bit_cast(f16_t, nb) : h;
+        const float s = __uint_as_float(a) * 16777216.0f;  // exact
+        const float t = __builtin_floorf(s + (float)(r >> 8) * (1.0f / 16777216.0f));
+        return (f16_t)__uint_as_float(__float_as_uint(t * 5.9604644775390625e-8f) | sign);
     }
 };
 template <>
@@ -124,7 +126,7 @@ struct Rounder<bf16_t, 1> {
     {
         uint32_t u = __float_as_uint(v);
         if ((u & 0x7f800000U) == 0x7f800000U) return bf16_t{(uint16_t)(u >> 16)};
-        u += (r & 0xffffU);  // add a uniform fraction of one bf16 ulp, then truncate
+        u += (r >> 16);  // add a uniform fraction of one bf16 ulp (top bits of r), then truncate
         return bf16_t{(uint16_t)(u >> 16)};
     }
 };
