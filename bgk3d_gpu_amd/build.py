@@ -36,7 +36,7 @@ TARGETS = {
     "libbgk3d_gpu_sp.so": (["v1_abi.hip"], []),
     "libbgk3d_gpu_dp.so": (["v1_abi.hip"], ["-DBGK_DOUBLE"]),
 }
-ENGINE_SOURCES = ["engine.hip"]
+ENGINE_SOURCES = ["engine.hip", "serving.hip"]
 if (CSRC / "engine.hip").exists():
     TARGETS["libbgk3d_engine.so"] = (ENGINE_SOURCES, [])
 

@@ -150,20 +150,44 @@ inline RowMap row_map(int span)
     return r;
 }
 
+// Block-capped variant for launches whose block size is fixed at
+// 64 * maxw threads (the batched serving kernel): rows longer than maxw
+// waves are split into segments of maxw waves, shorter rows are packed
+// maxw / wpr to a block (a block may then hold idle waves).
+inline RowMap row_map_capped(int span, int maxw)
+{
+    RowMap r;
+    r.wpr = ceil_div(span, 64);
+    if (r.wpr > maxw) {
+        r.wpb = maxw;
+        r.rpb = 1;
+        r.bpr = ceil_div(r.wpr, maxw);
+    } else {
+        r.rpb = maxw / r.wpr;
+        r.wpb = r.rpb * r.wpr;
+        r.bpr = 1;
+    }
+    return r;
+}
+
+// The work of one block (bx, by, bz) of the launch geometry `rm` describes:
+// pull, collide, store.  `seed` is the step's stochastic-rounding seed.
 template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
-__global__ __launch_bounds__(512) void collide_fused(FusedArgs<S> p, RowMap rm)
+__device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& rm, int bx, int by, int bz,
+                                            uint32_t seed)
 {
     typedef typename Storage<S>::compute C;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave >= rm.wpb) return;
     int seg, j;
     if (rm.rpb == 1) {
-        seg = (int)blockIdx.x * rm.wpb + wave;
-        j = 1 + (int)blockIdx.y;
+        seg = bx * rm.wpb + wave;
+        j = 1 + by;
     } else {
         seg = wave % rm.wpr;
-        j = 1 + (int)blockIdx.y * rm.rpb + wave / rm.wpr;
+        j = 1 + by * rm.rpb + wave / rm.wpr;
     }
-    const int k = 1 + blockIdx.z;
+    const int k = 1 + bz;
     if (j > p.m) return;
     int shift = 0;
     if (p.align) {
@@ -193,13 +217,19 @@ __global__ __launch_bounds__(512) void collide_fused(FusedArgs<S> p, RowMap rm)
     // stochastic rounding: one hash per cell and step, then one LCG step per
     // store (deterministic: independent of scheduling)
     uint32_t r = 0;
-    if (SR) r = hash32(p.seed ^ ((uint32_t)s * 0x9e3779b1u));
+    if (SR) r = hash32(seed ^ ((uint32_t)s * 0x9e3779b1u));
 #pragma unroll
     for (int d = 0; d < 19; ++d) {
         S* dst = (d < 18 ? p.b[d] : p.a[18]) + s;
         if (SR) r = lcg_next(r);
         store_nt(dst, Rounder<S, SR>::cvt(o[d], r));
     }
+}
+
+template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
+__global__ __launch_bounds__(512) void collide_fused(FusedArgs<S> p, RowMap rm)
+{
+    fused_rows<S, BX, BY, BZ, OBST, FMODE, SR>(p, rm, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, p.seed);
 }
 
 // Launch geometry shared by both ABIs; `span` is the row length the waves

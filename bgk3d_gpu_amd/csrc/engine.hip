@@ -10,7 +10,7 @@
 // stores then touch whole lines.  Direction 19 is updated in place and lives
 // only in lattice 0; directions 1..18 alternate between the two lattices
 // (the reference's pointer swap, SRC/col_MC.F90:539-596), tracked by `cur`.
-#include "../../include/bgk3d_engine.h"
+#include "engine_internal.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -19,19 +19,17 @@
 #include <string>
 #include <vector>
 
-#include "collide.hpp"
-#include "common.hpp"
-#include "d3q19.hpp"
 #include "host_init.hpp"
-#include "reduce.hpp"
-#include "storage.hpp"
-#include "v1_kernels.hpp"
 
 using namespace bgk;
+using namespace bgk::eng;
+
+namespace bgk {
+namespace eng {
 
 namespace {
-
 thread_local std::string g_err;
+}
 
 int fail(int code, const char* fmt, ...)
 {
@@ -44,76 +42,11 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
-#define ENG_HIP(call)                                                                                    \
-    do {                                                                                                 \
-        hipError_t _e = (call);                                                                          \
-        if (_e != hipSuccess) return fail(BGK_EHIP, "%s: %s", #call, hipGetErrorString(_e));              \
-    } while (0)
+const char* last_error() { return g_err.c_str(); }
 
-size_t storage_bytes(int kind)
-{
-    switch (kind) {
-        case BGK_F32: return 4;
-        case BGK_F64: return 8;
-        default: return 2;
-    }
-}
+}  // namespace eng
+}  // namespace bgk
 
-long round_up(long v, long a) { return (v + a - 1) / a * a; }
-
-}  // namespace
-
-struct bgk_handle {
-    bgk_config cfg{};
-    size_t esize = 4;
-    int A = 32;  // elements per 128-byte line
-    int nx = 0, ny = 0, nz = 0;
-    long nxy = 0, astride = 0, base = 0;
-    void* mem[2] = {nullptr, nullptr};
-    size_t bytes = 0;
-    int cur = 0;
-    hipStream_t stream = nullptr;
-    double omega = 0, fgrad = 0, lidforce = 0, u_inflow = 0;
-    // obstacle
-    bool has_obs = false;
-    int ic = 0, jc = 0, kc = 0, radius = 0;
-    int ob[6] = {0, 0, 0, 0, 0, 0};  // link-mask box origin (i,j,k) and extent
-    uint32_t* obits = nullptr;
-    int mk[6] = {0, 0, 0, 0, 0, 0};  // solid-mask box origin and extent (draglift)
-    int8_t* omask = nullptr;
-    double* scratch = nullptr;
-    double* hscratch = nullptr;
-    long long steps = 0;
-    std::vector<hipEvent_t> events;
-
-    template <typename S>
-    S* arr(int buf, int d) const
-    {
-        return reinterpret_cast<S*>(mem[buf]) + (long)d * astride + base;
-    }
-    template <typename S>
-    S* cur_a(int d) const
-    {
-        return d == 18 ? arr<S>(0, 18) : arr<S>(cur, d);
-    }
-    template <typename S>
-    Lattice19<S> current() const
-    {
-        Lattice19<S> L;
-        for (int d = 0; d < 19; ++d) L.p[d] = cur_a<S>(d);
-        return L;
-    }
-    Geometry geom() const
-    {
-        Geometry g;
-        g.l = cfg.l;
-        g.m = cfg.m;
-        g.n = cfg.n;
-        g.nx = nx;
-        g.nxy = nxy;
-        return g;
-    }
-};
 
 namespace {
 
@@ -310,29 +243,7 @@ int upload_state(bgk_handle& h)
 template <typename S, int BX, int BY, int BZ, bool OB, int FM, int SR>
 void launch_fused(bgk_handle& h)
 {
-    typedef typename Storage<S>::compute C;
-    FusedArgs<S> p{};
-    for (int d = 0; d < 19; ++d) p.a[d] = h.cur_a<S>(d);
-    for (int d = 0; d < 18; ++d) p.b[d] = h.arr<S>(1 - h.cur, d);
-    p.l = h.cfg.l;
-    p.m = h.cfg.m;
-    p.n = h.cfg.n;
-    p.nx = h.nx;
-    p.nxy = h.nxy;
-    p.omega = (C)h.omega;
-    p.fgrad = (C)h.fgrad;
-    p.fx = p.fy = p.fz = C(0);
-    p.lidforce = (C)h.lidforce;
-    p.cst = CollideConsts<C>::shifted();
-    p.obits = h.obits;
-    p.oi0 = h.ob[0];
-    p.oj0 = h.ob[1];
-    p.ok0 = h.ob[2];
-    p.oni = h.ob[3];
-    p.onj = h.ob[4];
-    p.onk = h.ob[5];
-    p.align = 0;
-    p.seed = hash32(h.cfg.seed ^ hash32((uint32_t)h.steps * 0x9e3779b9U + 1u));
+    const FusedArgs<S> p = fused_args<S>(h, h.cur);
     const RowMap rm = row_map(p.l);
     hipLaunchKernelGGL((collide_fused<S, BX, BY, BZ, OB, FM, SR>), fused_grid(rm, p.m, p.n), fused_block(rm), 0,
                        h.stream, p, rm);
@@ -385,18 +296,6 @@ void one_step(bgk_handle& h, hipEvent_t pre, hipEvent_t post)
     ++h.steps;
 }
 
-template <typename F>
-int by_storage(int kind, F&& f)
-{
-    switch (kind) {
-        case BGK_F32: return f(float{});
-        case BGK_F64: return f(double{});
-        case BGK_F16: return f(f16_t{});
-        case BGK_BF16: return f(bf16_t{});
-    }
-    return fail(BGK_EINVAL, "unknown storage type %d", kind);
-}
-
 template <typename S>
 typename Storage<S>::compute cte_one()
 {
@@ -404,6 +303,53 @@ typename Storage<S>::compute cte_one()
 }
 
 }  // namespace
+
+namespace bgk {
+namespace eng {
+
+int handle_setup(bgk_handle& h, const bgk_config& c)
+{
+    if (c.l < 1 || c.m < 1 || c.n < 1) return fail(BGK_EINVAL, "grid %d x %d x %d", c.l, c.m, c.n);
+    if (c.boundary < 0 || c.boundary > 3) return fail(BGK_EINVAL, "boundary %d", c.boundary);
+    if (c.storage < 0 || c.storage > 3) return fail(BGK_EINVAL, "storage %d", c.storage);
+    if (c.init != 0 && c.init != 1) return fail(BGK_EINVAL, "init %d", c.init);
+    if (c.obstacle < 0 || c.obstacle > 2) return fail(BGK_EINVAL, "obstacle %d", c.obstacle);
+    if (c.obstacle && c.boundary != BGK_BC_INFLOW && c.boundary != BGK_BC_PERIODIC)
+        return fail(BGK_EINVAL, "obstacles are supported with inflow or periodic boundaries");
+    if (!(c.svisc > 0.0)) return fail(BGK_EINVAL, "svisc must be positive");
+    if (c.force_axis < 1 || c.force_axis > 3) return fail(BGK_EINVAL, "force_axis %d", c.force_axis);
+    h.cfg = c;
+    h.esize = storage_bytes(c.storage);
+    h.A = (int)(128 / h.esize);
+    h.nx = (int)round_up(c.l + 2, h.A);
+    h.ny = c.m + 2;
+    h.nz = c.n + 2;
+    h.nxy = (long)h.nx * h.ny;
+    h.base = h.A - 1;
+    h.astride = round_up(h.A + h.nxy * h.nz, h.A);
+    // derived parameters in the collision kind (SRC/hencol.F90:31, :52;
+    // SRC/bcond.driven.F90:61; SRC/build.bcond.F90:29-35)
+    const bool dbl = c.storage == BGK_F64;
+    auto K = [&](double v) { return dbl ? v : (double)(float)v; };
+    const double nu = K(c.svisc), u0f = K(c.boundary == BGK_BC_CHANNEL ? c.u0 : 0.0),
+                 u00 = K(c.boundary == BGK_BC_CHANNEL ? 0.0 : c.u0);
+    if (dbl) {
+        h.omega = (1.0 + 1.0) / ((6.0 * nu) + 1.0);
+        h.fgrad = 4.0 * u0f * nu / (5.0 * (double)c.m * (double)c.m);
+        h.lidforce = u00 / 6.0;
+    } else {
+        const float nuf = (float)nu;
+        h.omega = (1.0f + 1.0f) / ((6.0f * nuf) + 1.0f);
+        h.fgrad = 4.0f * (float)u0f * nuf / (5.0f * (float)c.m * (float)c.m);
+        h.lidforce = (float)u00 / 6.0f;
+    }
+    h.u_inflow = c.u_inflow < 0 ? (double)0.1f : K(c.u_inflow);
+
+    return 0;
+}
+
+}  // namespace eng
+}  // namespace bgk
 
 extern "C" {
 
@@ -422,54 +368,27 @@ void bgk_config_default(bgk_config* c, int l, int m, int n)
     c->force_axis = 1;
 }
 
-const char* bgk_last_error(void) { return g_err.c_str(); }
+const char* bgk_last_error(void) { return last_error(); }
 
 int bgk_create(const bgk_config* cfg, bgk_handle** out)
 {
     if (!cfg || !out) return fail(BGK_EINVAL, "null argument");
     *out = nullptr;
     const bgk_config& c = *cfg;
-    if (c.l < 1 || c.m < 1 || c.n < 1) return fail(BGK_EINVAL, "grid %d x %d x %d", c.l, c.m, c.n);
-    if (c.boundary < 0 || c.boundary > 3) return fail(BGK_EINVAL, "boundary %d", c.boundary);
-    if (c.storage < 0 || c.storage > 3) return fail(BGK_EINVAL, "storage %d", c.storage);
-    if (c.init != 0 && c.init != 1) return fail(BGK_EINVAL, "init %d", c.init);
-    if (c.obstacle < 0 || c.obstacle > 2) return fail(BGK_EINVAL, "obstacle %d", c.obstacle);
-    if (c.obstacle && c.boundary != BGK_BC_INFLOW && c.boundary != BGK_BC_PERIODIC)
-        return fail(BGK_EINVAL, "obstacles are supported with inflow or periodic boundaries");
-    if (!(c.svisc > 0.0)) return fail(BGK_EINVAL, "svisc must be positive");
-    if (c.force_axis < 1 || c.force_axis > 3) return fail(BGK_EINVAL, "force_axis %d", c.force_axis);
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || c.device < 0 || c.device >= ndev)
-        return fail(BGK_EINVAL, "device %d not available (%d devices)", c.device, ndev);
-    ENG_HIP(hipSetDevice(c.device));
-
     bgk_handle* h = new bgk_handle();
-    h->cfg = c;
-    h->esize = storage_bytes(c.storage);
-    h->A = (int)(128 / h->esize);
-    h->nx = (int)round_up(c.l + 2, h->A);
-    h->ny = c.m + 2;
-    h->nz = c.n + 2;
-    h->nxy = (long)h->nx * h->ny;
-    h->base = h->A - 1;
-    h->astride = round_up(h->A + h->nxy * h->nz, h->A);
-    // derived parameters in the collision kind (SRC/hencol.F90:31, :52;
-    // SRC/bcond.driven.F90:61; SRC/build.bcond.F90:29-35)
-    const bool dbl = c.storage == BGK_F64;
-    auto K = [&](double v) { return dbl ? v : (double)(float)v; };
-    const double nu = K(c.svisc), u0f = K(c.boundary == BGK_BC_CHANNEL ? c.u0 : 0.0),
-                 u00 = K(c.boundary == BGK_BC_CHANNEL ? 0.0 : c.u0);
-    if (dbl) {
-        h->omega = (1.0 + 1.0) / ((6.0 * nu) + 1.0);
-        h->fgrad = 4.0 * u0f * nu / (5.0 * (double)c.m * (double)c.m);
-        h->lidforce = u00 / 6.0;
-    } else {
-        const float nuf = (float)nu;
-        h->omega = (1.0f + 1.0f) / ((6.0f * nuf) + 1.0f);
-        h->fgrad = 4.0f * (float)u0f * nuf / (5.0f * (float)c.m * (float)c.m);
-        h->lidforce = (float)u00 / 6.0f;
+    if (int rc = handle_setup(*h, c)) {
+        delete h;
+        return rc;
     }
-    h->u_inflow = c.u_inflow < 0 ? (double)0.1f : K(c.u_inflow);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || c.device < 0 || c.device >= ndev) {
+        delete h;
+        return fail(BGK_EINVAL, "device %d not available (%d devices)", c.device, ndev);
+    }
+    if (hipSetDevice(c.device) != hipSuccess) {
+        delete h;
+        return fail(BGK_EHIP, "hipSetDevice(%d) failed", c.device);
+    }
 
     int rc = 0;
     auto cleanup = [&](int code) {
@@ -478,12 +397,11 @@ int bgk_create(const bgk_config* cfg, bgk_handle** out)
     };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(BGK_EHIP, "stream creation failed"));
-    const size_t bytes0 = (size_t)h->astride * 19 * h->esize, bytes1 = (size_t)h->astride * 18 * h->esize;
-    if (hipMalloc(&h->mem[0], bytes0) != hipSuccess || hipMalloc(&h->mem[1], bytes1) != hipSuccess)
-        return cleanup(fail(BGK_ENOMEM, "cannot allocate %.2f GB of lattice", (bytes0 + bytes1) / 1e9));
-    h->bytes = bytes0 + bytes1;
-    hipMemsetAsync(h->mem[0], 0, bytes0, h->stream);
-    hipMemsetAsync(h->mem[1], 0, bytes1, h->stream);
+    void* region = nullptr;
+    if (hipMalloc(&region, lattice_bytes(*h)) != hipSuccess)
+        return cleanup(fail(BGK_ENOMEM, "cannot allocate %.2f GB of lattice", lattice_bytes(*h) / 1e9));
+    place_lattices(*h, region);
+    hipMemsetAsync(region, 0, lattice_bytes(*h), h->stream);
 
     // obstacle: reference sphere shell by default (SRC/build.obs.F90:52-60)
     if (c.obstacle) {
@@ -554,13 +472,14 @@ int bgk_destroy(bgk_handle* h)
     if (!h) return 0;
     if (h->stream) hipStreamSynchronize(h->stream);
     for (auto e : h->events) hipEventDestroy(e);
-    if (h->mem[0]) hipFree(h->mem[0]);
-    if (h->mem[1]) hipFree(h->mem[1]);
     if (h->obits) hipFree(h->obits);
     if (h->omask) hipFree(h->omask);
-    if (h->scratch) hipFree(h->scratch);
-    if (h->hscratch) hipHostFree(h->hscratch);
-    if (h->stream) hipStreamDestroy(h->stream);
+    if (!h->borrowed) {
+        if (h->mem[0]) hipFree(h->mem[0]);  // one region holds both lattices
+        if (h->scratch) hipFree(h->scratch);
+        if (h->hscratch) hipHostFree(h->hscratch);
+        if (h->stream) hipStreamDestroy(h->stream);
+    }
     delete h;
     return 0;
 }

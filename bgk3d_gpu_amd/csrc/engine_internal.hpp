@@ -1,0 +1,160 @@
+// Internals shared by the handle API (engine.hip) and the serving layer
+// (serving.hip): the handle, its layout, the derived collision parameters
+// and the arguments of the fused kernel for one step.
+#pragma once
+
+#include "../../include/bgk3d_engine.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "collide.hpp"
+#include "common.hpp"
+#include "d3q19.hpp"
+#include "reduce.hpp"
+#include "storage.hpp"
+#include "v1_kernels.hpp"
+
+struct bgk_handle {
+    bgk_config cfg{};
+    size_t esize = 4;
+    int A = 32;  // elements per 128-byte line
+    int nx = 0, ny = 0, nz = 0;
+    long nxy = 0, astride = 0, base = 0;
+    void* mem[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+    int cur = 0;
+    hipStream_t stream = nullptr;
+    double omega = 0, fgrad = 0, lidforce = 0, u_inflow = 0;
+    // obstacle
+    bool has_obs = false;
+    int ic = 0, jc = 0, kc = 0, radius = 0;
+    int ob[6] = {0, 0, 0, 0, 0, 0};  // link-mask box origin (i,j,k) and extent
+    uint32_t* obits = nullptr;
+    int mk[6] = {0, 0, 0, 0, 0, 0};  // solid-mask box origin and extent (draglift)
+    int8_t* omask = nullptr;
+    double* scratch = nullptr;
+    double* hscratch = nullptr;
+    long long steps = 0;
+    bool borrowed = false;  // lattice memory, scratch and stream belong to a server
+    std::vector<hipEvent_t> events;
+
+    template <typename S>
+    S* arr(int buf, int d) const
+    {
+        return reinterpret_cast<S*>(mem[buf]) + (long)d * astride + base;
+    }
+    template <typename S>
+    S* cur_a(int d) const
+    {
+        return d == 18 ? arr<S>(0, 18) : arr<S>(cur, d);
+    }
+    template <typename S>
+    bgk::Lattice19<S> current() const
+    {
+        bgk::Lattice19<S> L;
+        for (int d = 0; d < 19; ++d) L.p[d] = cur_a<S>(d);
+        return L;
+    }
+    bgk::Geometry geom() const
+    {
+        bgk::Geometry g;
+        g.l = cfg.l;
+        g.m = cfg.m;
+        g.n = cfg.n;
+        g.nx = nx;
+        g.nxy = nxy;
+        return g;
+    }
+};
+
+namespace bgk {
+namespace eng {
+
+// Record an error message for bgk_last_error() and return `code`.
+int fail(int code, const char* fmt, ...);
+
+#define ENG_HIP(call)                                                                                    \
+    do {                                                                                                 \
+        hipError_t _e = (call);                                                                          \
+        if (_e != hipSuccess) return ::bgk::eng::fail(BGK_EHIP, "%s: %s", #call, hipGetErrorString(_e)); \
+    } while (0)
+
+inline size_t storage_bytes(int kind)
+{
+    switch (kind) {
+        case BGK_F32: return 4;
+        case BGK_F64: return 8;
+        default: return 2;
+    }
+}
+
+inline long round_up(long v, long a) { return (v + a - 1) / a * a; }
+
+template <typename F>
+int by_storage(int kind, F&& f)
+{
+    switch (kind) {
+        case BGK_F32: return f(float{});
+        case BGK_F64: return f(double{});
+        case BGK_F16: return f(f16_t{});
+        case BGK_BF16: return f(bf16_t{});
+    }
+    return fail(BGK_EINVAL, "unknown storage type %d", kind);
+}
+
+// Validate `c` and fill the layout and the derived parameters of `h`
+// (no device call, no allocation).
+int handle_setup(bgk_handle& h, const bgk_config& c);
+
+// Device bytes of the two lattices of a handle set up by handle_setup.
+inline size_t lattice_bytes(const bgk_handle& h) { return (size_t)h.astride * 37 * h.esize; }
+
+// Point mem[0] (19 arrays) and mem[1] (18 arrays) into one region of
+// lattice_bytes(h) bytes.
+inline void place_lattices(bgk_handle& h, void* region)
+{
+    h.mem[0] = region;
+    h.mem[1] = static_cast<char*>(region) + (size_t)h.astride * 19 * h.esize;
+    h.bytes = lattice_bytes(h);
+}
+
+// Stochastic-rounding seed of the step that follows `steps` completed steps.
+__host__ __device__ inline uint32_t step_seed(uint32_t cfg_seed, long long steps)
+{
+    return hash32(cfg_seed ^ hash32((uint32_t)steps * 0x9e3779b9U + 1u));
+}
+
+// Arguments of the fused kernel for the step that reads lattice `cur`.
+template <typename S>
+FusedArgs<S> fused_args(const bgk_handle& h, int cur)
+{
+    typedef typename Storage<S>::compute C;
+    FusedArgs<S> p{};
+    for (int d = 0; d < 19; ++d) p.a[d] = d == 18 ? h.arr<S>(0, 18) : h.arr<S>(cur, d);
+    for (int d = 0; d < 18; ++d) p.b[d] = h.arr<S>(1 - cur, d);
+    p.l = h.cfg.l;
+    p.m = h.cfg.m;
+    p.n = h.cfg.n;
+    p.nx = h.nx;
+    p.nxy = h.nxy;
+    p.omega = (C)h.omega;
+    p.fgrad = (C)h.fgrad;
+    p.fx = p.fy = p.fz = C(0);
+    p.lidforce = (C)h.lidforce;
+    p.cst = CollideConsts<C>::shifted();
+    p.obits = h.obits;
+    p.oi0 = h.ob[0];
+    p.oj0 = h.ob[1];
+    p.ok0 = h.ob[2];
+    p.oni = h.ob[3];
+    p.onj = h.ob[4];
+    p.onk = h.ob[5];
+    p.align = 0;
+    p.seed = step_seed(h.cfg.seed, h.steps);
+    return p;
+}
+
+}  // namespace eng
+}  // namespace bgk

@@ -1,0 +1,509 @@
+// Continuous-batching server (bgks_* in include/bgk3d_engine.h): many
+// independent lid-driven cavity solves of different sizes, viscosities and
+// storage types advanced together on one GPU.
+//
+// The reference runs such a sweep as separate programs, one after the other
+// (UTIL/run_sweep.sh:55-63, 128-169), each with its own time loop
+// (SRC/bgk3d.F90:169-196).  Here every admitted request is an engine handle
+// whose two lattices live in one device arena sized by the caller's HBM
+// budget; requests are admitted in arrival order while the arena can hold
+// them, and all active requests are advanced by one kernel launch per
+// (storage type, rounding) group and step.  The launch grid is the
+// concatenation of every request's row blocks; a block finds its request
+// through a per-block owner table and then runs the same per-cell code as
+// the single-request kernel, so a request's result does not depend on what
+// it shares the GPU with (stochastic-rounding streams are keyed by cell
+// index, request seed and the request's own step count).
+//
+// Steps run in chunks of at most `chunk_steps`, shortened so that no request
+// overshoots its step count; requests join and leave only between chunks.
+#include "engine_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+using namespace bgk;
+using namespace bgk::eng;
+
+namespace {
+
+constexpr int kServeWaves = 4;  // block = 256 threads for every request
+constexpr size_t kArenaAlign = 64 << 10;
+
+template <typename S>
+struct BatchSlot {
+    FusedArgs<S> args[2];  // indexed by the lattice the step reads
+    RowMap rm;
+    int gy;                // row blocks along j
+    int par0;              // lattice read at chunk step 0
+    uint32_t cfg_seed;
+    long long steps0;      // steps completed before the chunk
+    uint32_t block0;       // first block of this request in the launch
+};
+
+// One time step of every request of one storage group.  The lid-driven
+// cavity: bounce-back walls in x and y, lid at z = n + 1 (SRC/bcond.driven.F90).
+template <typename S, int SR>
+__global__ __launch_bounds__(64 * kServeWaves) void collide_batched(const BatchSlot<S>* __restrict__ slots,
+                                                                    const uint32_t* __restrict__ owner, int t)
+{
+    const uint32_t b = blockIdx.x;
+    const BatchSlot<S>& sl = slots[owner[b]];
+    const RowMap rm = sl.rm;
+    const uint32_t lb = b - sl.block0;
+    const int bx = (int)(lb % (uint32_t)rm.bpr);
+    const uint32_t rest = lb / (uint32_t)rm.bpr;
+    const int by = (int)(rest % (uint32_t)sl.gy);
+    const int bz = (int)(rest / (uint32_t)sl.gy);
+    const uint32_t seed = SR ? step_seed(sl.cfg_seed, sl.steps0 + t) : 0u;
+    fused_rows<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>(sl.args[(sl.par0 + t) & 1], rm, bx, by, bz, seed);
+}
+
+double clock_now()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over [0, size) with coalescing of free ranges.
+struct Arena {
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+
+    void reset(size_t n)
+    {
+        size = n;
+        free_.clear();
+        free_[0] = n;
+    }
+    bool alloc(size_t len, size_t& off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < len) continue;
+            off = it->first;
+            const size_t rest = it->second - len;
+            free_.erase(it);
+            if (rest) free_[off + len] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t len)
+    {
+        auto it = free_.emplace(off, len).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t used() const
+    {
+        size_t f = 0;
+        for (auto& kv : free_) f += kv.second;
+        return size - f;
+    }
+};
+
+struct Instance {
+    long long id = 0;
+    bgks_request req{};
+    bgk_handle h;
+    int state = BGKS_QUEUED;
+    bool has_mem = false;
+    size_t off = 0, len = 0;
+    double diag[5] = {0, 0, 0, 0, 0};
+    int unstable = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+int group_of(int storage, int rounding)
+{
+    const bool sr = rounding && (storage == BGK_F16 || storage == BGK_BF16);
+    return storage * 2 + (sr ? 1 : 0);
+}
+
+bgk_config request_config(const bgks_request& r, int device)
+{
+    bgk_config c;
+    bgk_config_default(&c, r.n, r.n, r.n);
+    c.boundary = BGK_BC_DRIVEN;
+    c.storage = r.storage;
+    c.rounding = r.rounding;
+    c.svisc = r.svisc;
+    c.u0 = r.u0;
+    c.device = device;
+    c.seed = r.seed;
+    return c;
+}
+
+}  // namespace
+
+struct bgks_server {
+    int device = 0;
+    int chunk = 100;
+    hipStream_t stream = nullptr;
+    void* arena_mem = nullptr;
+    Arena arena;
+    double* scratch = nullptr;
+    double* hscratch = nullptr;
+    std::deque<long long> queue;
+    std::map<long long, std::unique_ptr<Instance>> all;
+    long long next_id = 1;
+    // per storage group: device tables and pinned staging
+    struct Group {
+        void* d_slots = nullptr;
+        uint32_t* d_owner = nullptr;
+        void* h_slots = nullptr;
+        uint32_t* h_owner = nullptr;
+        size_t slot_bytes_cap = 0, owner_cap = 0;
+    } groups[8];
+    long long launches = 0, cell_steps = 0, chunks = 0;
+};
+
+namespace {
+
+int grow_group(bgks_server& s, bgks_server::Group& g, size_t slot_bytes, size_t owners)
+{
+    if (slot_bytes > g.slot_bytes_cap) {
+        if (g.d_slots) hipFree(g.d_slots);
+        if (g.h_slots) hipHostFree(g.h_slots);
+        g.d_slots = g.h_slots = nullptr;
+        const size_t cap = std::max(slot_bytes, 2 * g.slot_bytes_cap);
+        ENG_HIP(hipMalloc(&g.d_slots, cap));
+        ENG_HIP(hipHostMalloc(&g.h_slots, cap, hipHostMallocDefault));
+        g.slot_bytes_cap = cap;
+    }
+    if (owners > g.owner_cap) {
+        if (g.d_owner) hipFree(g.d_owner);
+        if (g.h_owner) hipHostFree(g.h_owner);
+        g.d_owner = g.h_owner = nullptr;
+        const size_t cap = std::max(owners, 2 * g.owner_cap);
+        ENG_HIP(hipMalloc(&g.d_owner, cap * sizeof(uint32_t)));
+        ENG_HIP(hipHostMalloc(&g.h_owner, cap * sizeof(uint32_t), hipHostMallocDefault));
+        g.owner_cap = cap;
+    }
+    (void)s;
+    return 0;
+}
+
+// Build and upload the launch tables of one storage group: a slot per
+// request (kernel arguments for both lattice parities, row mapping, seed,
+// step count at the chunk start) and the owner of every block.
+template <typename S>
+int prepare_group(bgks_server& s, bgks_server::Group& g, const std::vector<Instance*>& members, uint32_t& nblocks)
+{
+    std::vector<BatchSlot<S>> slots(members.size());
+    uint32_t total = 0;
+    for (size_t q = 0; q < members.size(); ++q) {
+        bgk_handle& h = members[q]->h;
+        BatchSlot<S>& sl = slots[q];
+        sl.args[0] = fused_args<S>(h, 0);
+        sl.args[1] = fused_args<S>(h, 1);
+        sl.rm = row_map_capped(h.cfg.l, kServeWaves);
+        sl.gy = ceil_div(h.cfg.m, sl.rm.rpb);
+        sl.par0 = h.cur;
+        sl.cfg_seed = h.cfg.seed;
+        sl.steps0 = h.steps;
+        sl.block0 = total;
+        const long long nb = (long long)sl.rm.bpr * sl.gy * h.cfg.n;
+        if ((long long)total + nb > 0xffffffffLL) return fail(BGK_EINVAL, "batch grid exceeds 2^32 blocks");
+        total += (uint32_t)nb;
+    }
+    if (int rc = grow_group(s, g, slots.size() * sizeof(BatchSlot<S>), total)) return rc;
+    std::memcpy(g.h_slots, slots.data(), slots.size() * sizeof(BatchSlot<S>));
+    for (size_t q = 0; q < members.size(); ++q) {
+        const uint32_t b0 = slots[q].block0;
+        const uint32_t b1 = q + 1 < members.size() ? slots[q + 1].block0 : total;
+        std::fill(g.h_owner + b0, g.h_owner + b1, (uint32_t)q);
+    }
+    ENG_HIP(hipMemcpyAsync(g.d_slots, g.h_slots, slots.size() * sizeof(BatchSlot<S>), hipMemcpyHostToDevice,
+                           s.stream));
+    ENG_HIP(hipMemcpyAsync(g.d_owner, g.h_owner, (size_t)total * sizeof(uint32_t), hipMemcpyHostToDevice, s.stream));
+    nblocks = total;
+    return 0;
+}
+
+template <typename S, int SR>
+void launch_group(bgks_server& s, bgks_server::Group& g, uint32_t nblocks, int t)
+{
+    hipLaunchKernelGGL((collide_batched<S, SR>), dim3(nblocks), dim3(64 * kServeWaves), 0, s.stream,
+                       static_cast<const BatchSlot<S>*>(g.d_slots), g.d_owner, t);
+}
+
+template <typename F>
+int by_group(int gid, F&& f)
+{
+    switch (gid) {
+        case 0: return f(float{}, std::integral_constant<int, 0>{});
+        case 2: return f(double{}, std::integral_constant<int, 0>{});
+        case 4: return f(f16_t{}, std::integral_constant<int, 0>{});
+        case 5: return f(f16_t{}, std::integral_constant<int, 1>{});
+        case 6: return f(bf16_t{}, std::integral_constant<int, 0>{});
+        case 7: return f(bf16_t{}, std::integral_constant<int, 1>{});
+    }
+    return fail(BGK_EINVAL, "storage group %d", gid);
+}
+
+int admit(bgks_server& s)
+{
+    while (!s.queue.empty()) {
+        Instance& in = *s.all.at(s.queue.front());
+        size_t off = 0;
+        if (!s.arena.alloc(in.len, off)) break;  // arrival order: the head waits for room
+        in.off = off;
+        in.has_mem = true;
+        void* region = static_cast<char*>(s.arena_mem) + off;
+        place_lattices(in.h, region);
+        ENG_HIP(hipMemsetAsync(region, 0, in.len, s.stream));  // rest state: shifted populations are zero
+        in.state = BGKS_ACTIVE;
+        in.t_admit = clock_now();
+        s.queue.pop_front();
+    }
+    return 0;
+}
+
+void free_lattice(bgks_server& s, Instance& in)
+{
+    if (!in.has_mem) return;
+    s.arena.release(in.off, in.len);
+    in.has_mem = false;
+    in.h.mem[0] = in.h.mem[1] = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+double bgks_clock(void) { return clock_now(); }
+
+int bgks_request_bytes(const bgks_request* r, size_t* bytes)
+{
+    if (!r || !bytes) return fail(BGK_EINVAL, "null argument");
+    bgk_handle h;
+    if (int rc = handle_setup(h, request_config(*r, 0))) return rc;
+    *bytes = round_up((long)lattice_bytes(h), (long)kArenaAlign);
+    return 0;
+}
+
+int bgks_create(int device, size_t budget_bytes, int chunk_steps, bgks_server** out)
+{
+    if (!out) return fail(BGK_EINVAL, "null argument");
+    *out = nullptr;
+    if (chunk_steps < 1) return fail(BGK_EINVAL, "chunk_steps %d", chunk_steps);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(BGK_EINVAL, "device %d not available (%d devices)", device, ndev);
+    ENG_HIP(hipSetDevice(device));
+    budget_bytes = budget_bytes / kArenaAlign * kArenaAlign;
+    if (budget_bytes == 0) return fail(BGK_EINVAL, "budget below %zu bytes", kArenaAlign);
+    auto* s = new bgks_server();
+    s->device = device;
+    s->chunk = chunk_steps;
+    auto bail = [&](int code) {
+        bgks_destroy(s);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(BGK_EHIP, "stream creation failed"));
+    if (hipMalloc(&s->arena_mem, budget_bytes) != hipSuccess)
+        return bail(fail(BGK_ENOMEM, "cannot reserve %.2f GB for the request arena", budget_bytes / 1e9));
+    s->arena.reset(budget_bytes);
+    if (hipMalloc(&s->scratch, (kRedBlocks + 8) * 5 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&s->hscratch, (kRedBlocks + 8) * 5 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(BGK_ENOMEM, "scratch"));
+    *out = s;
+    return 0;
+}
+
+int bgks_destroy(bgks_server* s)
+{
+    if (!s) return 0;
+    hipSetDevice(s->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (auto& g : s->groups) {
+        if (g.d_slots) hipFree(g.d_slots);
+        if (g.d_owner) hipFree(g.d_owner);
+        if (g.h_slots) hipHostFree(g.h_slots);
+        if (g.h_owner) hipHostFree(g.h_owner);
+    }
+    if (s->arena_mem) hipFree(s->arena_mem);
+    if (s->scratch) hipFree(s->scratch);
+    if (s->hscratch) hipHostFree(s->hscratch);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return 0;
+}
+
+int bgks_submit(bgks_server* s, const bgks_request* r, long long* id)
+{
+    if (!s || !r || !id) return fail(BGK_EINVAL, "null argument");
+    if (r->steps < 0) return fail(BGK_EINVAL, "steps %lld", r->steps);
+    if (r->rounding != 0 && r->rounding != 1) return fail(BGK_EINVAL, "rounding %d", r->rounding);
+    auto in = std::make_unique<Instance>();
+    if (int rc = handle_setup(in->h, request_config(*r, s->device))) return rc;
+    in->h.borrowed = true;
+    in->h.stream = s->stream;
+    in->h.scratch = s->scratch;
+    in->h.hscratch = s->hscratch;
+    in->len = round_up((long)lattice_bytes(in->h), (long)kArenaAlign);
+    if (in->len > s->arena.size)
+        return fail(BGK_ENOMEM, "request needs %.3f GB, the budget is %.3f GB", in->len / 1e9, s->arena.size / 1e9);
+    in->req = *r;
+    in->id = s->next_id++;
+    in->t_submit = clock_now();
+    *id = in->id;
+    s->queue.push_back(in->id);
+    s->all[in->id] = std::move(in);
+    return 0;
+}
+
+int bgks_advance(bgks_server* s, long long* done_ids, int max_done, int* n_done)
+{
+    if (!s) return fail(BGK_EINVAL, "null server");
+    if (n_done) *n_done = 0;
+    ENG_HIP(hipSetDevice(s->device));
+    if (int rc = admit(*s)) return rc;
+
+    std::vector<Instance*> active;
+    for (auto& kv : s->all)
+        if (kv.second->state == BGKS_ACTIVE) active.push_back(kv.second.get());
+    long long K = s->chunk;
+    for (Instance* in : active) K = std::min(K, in->req.steps - in->h.steps);
+    K = std::max(K, 0LL);
+
+    if (K > 0 && !active.empty()) {
+        std::vector<Instance*> members[8];
+        for (Instance* in : active) members[group_of(in->h.cfg.storage, in->h.cfg.rounding)].push_back(in);
+        uint32_t nblocks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int g = 0; g < 8; ++g) {
+            if (members[g].empty()) continue;
+            int rc = by_group(g, [&](auto tag, auto) {
+                return prepare_group<decltype(tag)>(*s, s->groups[g], members[g], nblocks[g]);
+            });
+            if (rc) return rc;
+        }
+        for (long long t = 0; t < K; ++t)
+            for (int g = 0; g < 8; ++g) {
+                if (!nblocks[g]) continue;
+                by_group(g, [&](auto tag, auto sr) {
+                    launch_group<decltype(tag), decltype(sr)::value>(*s, s->groups[g], nblocks[g], (int)t);
+                    return 0;
+                });
+                ++s->launches;
+            }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(BGK_EHIP, "batched step launch: %s", hipGetErrorString(e));
+        for (Instance* in : active) {
+            in->h.steps += K;
+            in->h.cur = (int)((in->h.cur + K) & 1);
+            s->cell_steps += (long long)in->h.cfg.l * in->h.cfg.m * in->h.cfg.n * K;
+        }
+        ++s->chunks;
+    }
+    ENG_HIP(hipStreamSynchronize(s->stream));
+    const double now = clock_now();
+
+    int nd = 0;
+    for (Instance* in : active) {
+        if (in->h.steps < in->req.steps) continue;
+        if (int rc = bgk_diagno(&in->h, in->diag)) return rc;
+        in->unstable = 0;
+        for (double v : in->diag)
+            if (!std::isfinite(v)) in->unstable = 1;
+        in->state = BGKS_DONE;
+        in->t_done = now;
+        if (!in->req.keep) free_lattice(*s, *in);
+        if (done_ids && nd < max_done) done_ids[nd] = in->id;
+        ++nd;
+    }
+    if (n_done) *n_done = nd;
+    // requests that fit the memory just released start with the next chunk
+    return admit(*s);
+}
+
+int bgks_status(bgks_server* s, long long id, int* state, long long* steps_done)
+{
+    if (!s) return fail(BGK_EINVAL, "null server");
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return fail(BGK_EINVAL, "unknown request %lld", id);
+    if (state) *state = it->second->state;
+    if (steps_done) *steps_done = it->second->h.steps;
+    return 0;
+}
+
+int bgks_result(bgks_server* s, long long id, double diag[5], double times[3], int* unstable)
+{
+    if (!s) return fail(BGK_EINVAL, "null server");
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return fail(BGK_EINVAL, "unknown request %lld", id);
+    const Instance& in = *it->second;
+    if (in.state != BGKS_DONE) return fail(BGK_EBUSY, "request %lld has not finished", id);
+    if (diag) std::memcpy(diag, in.diag, sizeof in.diag);
+    if (times) {
+        times[0] = in.t_submit;
+        times[1] = in.t_admit;
+        times[2] = in.t_done;
+    }
+    if (unstable) *unstable = in.unstable;
+    return 0;
+}
+
+int bgks_get_population(bgks_server* s, long long id, int d, double* out)
+{
+    if (!s) return fail(BGK_EINVAL, "null server");
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return fail(BGK_EINVAL, "unknown request %lld", id);
+    Instance& in = *it->second;
+    if (!in.has_mem) return fail(BGK_EINVAL, "request %lld holds no lattice (not admitted, or released)", id);
+    ENG_HIP(hipStreamSynchronize(s->stream));
+    return bgk_get_population(&in.h, d, out);
+}
+
+int bgks_release(bgks_server* s, long long id)
+{
+    if (!s) return fail(BGK_EINVAL, "null server");
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return fail(BGK_EINVAL, "unknown request %lld", id);
+    Instance& in = *it->second;
+    if (in.state == BGKS_ACTIVE) return fail(BGK_EBUSY, "request %lld is running", id);
+    if (in.state == BGKS_QUEUED) s->queue.erase(std::find(s->queue.begin(), s->queue.end(), id));
+    free_lattice(*s, in);
+    s->all.erase(it);
+    return 0;
+}
+
+int bgks_stats(bgks_server* s, long long out[8])
+{
+    if (!s || !out) return fail(BGK_EINVAL, "null argument");
+    long long q = 0, a = 0, d = 0;
+    for (auto& kv : s->all) {
+        q += kv.second->state == BGKS_QUEUED;
+        a += kv.second->state == BGKS_ACTIVE;
+        d += kv.second->state == BGKS_DONE;
+    }
+    out[0] = q;
+    out[1] = a;
+    out[2] = d;
+    out[3] = (long long)s->arena.used();
+    out[4] = (long long)s->arena.size;
+    out[5] = s->launches;
+    out[6] = s->cell_steps;
+    out[7] = s->chunks;
+    return 0;
+}
+
+}  // extern "C"

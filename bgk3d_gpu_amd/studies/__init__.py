@@ -1,1 +1,1 @@
-"""Measurement studies of the BASELINE configurations (run on one GPU)."""
+"""Measurement studies: storage precision on a Poiseuille channel, flow past a sphere (one GPU)."""

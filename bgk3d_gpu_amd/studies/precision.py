@@ -1,6 +1,6 @@
 """Storage-precision study on the body-force Poiseuille channel.
 
-Configuration (BASELINE config 3): 512 x 128 x 128 cells, periodic in x and
+Configuration: 512 x 128 x 128 cells, periodic in x and
 y, half-way bounce-back walls at z = 1/2 and n + 1/2 (the reference's
 channel walls are in z, SRC/bcond.channel.F90:255-267; the flow is the same
 with the wall axis renamed), body force g_x = 1e-6 per step (10 * fgrad,

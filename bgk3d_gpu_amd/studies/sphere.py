@@ -1,4 +1,4 @@
-"""Flow past a sphere.
+"""Flow past a sphere: 1024 x 384 x 384 cells, Re = 300.
 
 1024 x 384 x 384 cells (151M), FP32, solid sphere of radius 32 centred at
 (256, 192, 192), inflow u = (0.05, 0, 0) at x = 0 and the reference's

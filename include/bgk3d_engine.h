@@ -5,8 +5,8 @@
  * hardware (rows padded so the first interior cell of every row starts a
  * 128-byte line), fold every boundary condition into the pull (no halo
  * kernels per step), store populations in FP32, FP64, FP16 or BF16 (shifted
- * f - w, collision in FP32 or FP64), and batch many small solves into one
- * persistent kernel (bgks_*).
+ * f - w, collision in FP32 or FP64), and advance many independent solves
+ * together, one launch per step for all of them (bgks_*).
  *
  * The reference has no such API: its driver owns the arrays and calls the
  * per-routine symbols of bgk3d_gpu.h (SRC/bgk3d.F90:169-196).  Each field
@@ -100,6 +100,74 @@ int bgk_info(bgk_handle* h, long long out[6]);
 
 /* Text of the last error on this thread. */
 const char* bgk_last_error(void);
+
+/* ---------------------------------------------------------------------------
+ * Continuous-batching server: many independent lid-driven cavity solves
+ * (each the LDC build of the reference, SRC/bcond.driven.F90, rest start)
+ * advanced together on one GPU.  The reference's analogue is a shell loop
+ * that runs one program per case (UTIL/run_sweep.sh:55-63, 128-169).
+ *
+ * Requests are admitted in arrival order while the device arena reserved by
+ * bgks_create (the HBM budget) can hold their two lattices; every active
+ * request advances by one kernel launch per (storage, rounding) group and
+ * step; requests join and leave between chunks of at most chunk_steps
+ * steps.  A request's populations are bit-identical to those of a handle
+ * (bgk_create with the same n, svisc, u0, storage, rounding and seed) after
+ * the same number of steps, whatever else is running.  Not thread-safe: one
+ * caller drives a server.
+ * ------------------------------------------------------------------------- */
+typedef struct bgks_server bgks_server;
+
+typedef struct bgks_request {
+    int n;               /* cavity n x n x n (namelist lx = ly = lz) */
+    double svisc;        /* viscosity; Re = u0 n / svisc (SRC/outdat.F90:46) */
+    double u0;           /* lid velocity (namelist u0) */
+    int storage;         /* enum bgk_storage */
+    int rounding;        /* 0 nearest-even, 1 stochastic (F16/BF16 only) */
+    long long steps;     /* time steps to run (namelist itfin) */
+    unsigned seed;       /* stochastic-rounding stream */
+    int keep;            /* 1: keep the lattice after completion, for
+                            bgks_get_population, until bgks_release */
+} bgks_request;
+
+enum bgks_state { BGKS_QUEUED = 0, BGKS_ACTIVE = 1, BGKS_DONE = 2 };
+
+/* Reserve `budget_bytes` of device memory for request lattices. */
+int bgks_create(int device, size_t budget_bytes, int chunk_steps, bgks_server** out);
+int bgks_destroy(bgks_server* s);
+
+/* Device bytes a request holds while admitted (both lattices, padded). */
+int bgks_request_bytes(const bgks_request* r, size_t* bytes);
+
+/* Queue a request; fails with BGK_ENOMEM if it can never fit the budget. */
+int bgks_submit(bgks_server* s, const bgks_request* r, long long* id);
+
+/* Admit what fits, run one chunk over all active requests, wait for it,
+ * retire the finished ones (device diagnostics, lattice freed unless keep)
+ * and admit again.  The ids finished by this call are written to done_ids
+ * (at most max_done of them); *n_done receives their number. */
+int bgks_advance(bgks_server* s, long long* done_ids, int max_done, int* n_done);
+
+/* enum bgks_state and steps completed. */
+int bgks_status(bgks_server* s, long long id, int* state, long long* steps_done);
+
+/* A finished request: diagnostics means as bgk_diagno, the times (seconds
+ * on the bgks_clock scale) of submission, admission and completion, and
+ * whether the solution became non-finite (unstable). */
+int bgks_result(bgks_server* s, long long id, double diag[5], double times[3], int* unstable);
+
+/* Interior values of population d of a request that holds its lattice. */
+int bgks_get_population(bgks_server* s, long long id, int d, double* out);
+
+/* Forget a finished or queued request and free its lattice. */
+int bgks_release(bgks_server* s, long long id);
+
+/* [0] queued, [1] active, [2] finished and not released, [3] arena bytes in
+ * use, [4] arena bytes, [5] kernel launches, [6] cell updates, [7] chunks. */
+int bgks_stats(bgks_server* s, long long out[8]);
+
+/* Monotonic clock (seconds) used for the request times. */
+double bgks_clock(void);
 
 #ifdef __cplusplus
 }

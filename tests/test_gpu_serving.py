@@ -1,0 +1,122 @@
+"""Continuous-batching server on the GPU.
+
+* Identity: every request of a mixed batch (sizes that give one, two and
+  three waves per row, FP32 / FP64 / FP16 / BF16, nearest and stochastic
+  rounding, different step counts so requests join and leave mid-run) ends
+  with populations bit-identical to the same request run alone on an engine
+  handle, and to the same batch run with another chunk length.
+* Admission: with a budget that holds two requests, no more than the budget
+  is ever in use, requests start in arrival order and all finish.
+* The diagnostics the server reports equal bgk_diagno of the alone run.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from bgk3d_gpu_amd import runtime as rt
+from bgk3d_gpu_amd import serving as S
+from bgk3d_gpu_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+MIX = [  # n, Re, storage, rounding, steps
+    (16, 100.0, "f32", 0, 120),
+    (40, 400.0, "f16", 1, 200),
+    (70, 1000.0, "f32", 0, 90),
+    (33, 250.0, "f64", 0, 150),
+    (130, 2000.0, "f16", 1, 40),
+    (24, 800.0, "bf16", 1, 170),
+    (47, 3200.0, "f16", 0, 130),
+    (65, 150.0, "bf16", 0, 60),
+]
+
+
+def mix_requests():
+    return [S.Request(n, re, st, steps, seed=17 + q, rounding=rnd, keep=True)
+            for q, (n, re, st, rnd, steps) in enumerate(MIX)]
+
+
+def serve(reqs, chunk, budget=8 << 30):
+    with S.Server(budget, chunk) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        done = set()
+        while len(done) < len(ids):
+            done.update(srv.advance())
+        pops = [srv.populations(i, r.n) for i, r in zip(ids, reqs)]
+        res = [srv.result(i) for i in ids]
+    return pops, res
+
+
+def alone(r):
+    with Engine(r.case(), r.storage, rounding=r.rounding if r.storage in ("f16", "bf16") else 0,
+                seed=r.seed) as eng:
+        eng.step(r.steps)
+        return eng.populations(), eng.diagno()
+
+
+def test_batched_requests_match_alone_runs(gpu):
+    reqs = mix_requests()
+    banner(f"serving identity: {len(reqs)} requests, chunk 64")
+    pops, res = serve(reqs, 64)
+    for q, r in enumerate(reqs):
+        banner(f"alone run {q}: n={r.n} {r.storage} rounding={r.rounding} steps={r.steps}")
+        p_alone, d_alone = alone(r)
+        assert np.array_equal(pops[q], p_alone, equal_nan=True), f"request {q} differs from its alone run"
+        assert np.array_equal(res[q]["diag"], d_alone, equal_nan=True)
+        assert not res[q]["unstable"]
+        assert np.all(np.isfinite(p_alone))
+        assert np.any(p_alone != 0.0)     # the lid moved the fluid
+
+
+def test_chunk_length_does_not_change_results(gpu):
+    reqs = mix_requests()[:5]
+    banner("serving chunk 7 vs chunk 1000")
+    a, _ = serve(reqs, 7)
+    b, _ = serve(reqs, 1000)
+    for q in range(len(reqs)):
+        assert np.array_equal(a[q], b[q], equal_nan=True)
+
+
+def test_admission_respects_the_budget(gpu):
+    reqs = [S.Request(n, 500.0, st, 60, seed=q) for q, (n, st) in
+            enumerate([(96, "f32"), (64, "f16"), (96, "f32"), (80, "f32"), (48, "f16")])]
+    need = [S.request_bytes(r) for r in reqs]
+    budget = max(need) * 2 + (1 << 16)
+    banner(f"serving admission: budget {budget} B, requests {need}")
+    with S.Server(budget, 25) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        done = []
+        peak = 0
+        while len(done) < len(ids):
+            st = srv.stats()
+            assert st["bytes_used"] <= st["budget"] == budget
+            peak = max(peak, st["active"])
+            done += srv.advance()
+            peak = max(peak, srv.stats()["active"])
+        res = [srv.result(i) for i in ids]
+        assert srv.stats()["bytes_used"] == 0
+    admits = [r["admit"] for r in res]
+    assert admits == sorted(admits), "requests must start in arrival order"
+    assert 1 <= peak <= 2
+    for r in res:
+        assert r["done"] >= r["admit"] >= r["submit"]
+        assert not r["unstable"]
+
+
+def test_request_larger_than_budget_is_refused(gpu):
+    r = S.Request(128, 500.0, "f32", 10)
+    with S.Server(S.request_bytes(r) // 2, 10) as srv:
+        with pytest.raises(rt.BGKError):
+            srv.submit(r)
+        small = S.Request(16, 500.0, "f32", 10)
+        rid = srv.submit(small)
+        while srv.status(rid)[0] != "done":
+            srv.advance()
+        assert srv.status(rid) == ("done", 10)
