@@ -5,7 +5,10 @@
 //   mode 0: rest flow -> every population is 0;
 //   mode 1: 3-D Taylor-Green, u = u00 sin x cos y cos z, v = -u00 cos x
 //           sin y cos z, w = 0, crho = 1 + u00^2/48 (cos 2x + cos 2y)
-//           (cos 2z + 2) in 2 pi units (:142-148).
+//           (cos 2z + 2) in 2 pi units (:142-148);
+//   mode 2: mode 1 with the density perturbation kept: the shifted form
+//           f - w = w (rho - 1) + w rho c, where the reference stores only
+//           w rho c (no reference counterpart).
 // Runs on the host with the C library's sin/cos of the working kind, like
 // the reference, so the state is bit-identical to the reference's.
 // Mixed-kind details kept: pi is a default-real literal (:63); the
@@ -25,7 +28,7 @@ namespace bgk {
 template <typename R>
 int init_feq_host(R** h, int l, int m, int n, int nx, int ny, int mode, R u00, int k0 = 0, int k1 = -1)
 {
-    if (mode != 0 && mode != 1) return -1;
+    if (mode < 0 || mode > 2) return -1;
     if (k1 < 0) k1 = n + 1;
     if (k0 < 0 || k1 > n + 1 || k0 > k1) return -1;
     const size_t nxy = (size_t)nx * (size_t)ny;
@@ -40,12 +43,13 @@ int init_feq_host(R** h, int l, int m, int n, int nx, int ny, int mode, R u00, i
             const R y = (R)(((double)(R)j - 0.5) / (double)(R)m);
             for (int i = 0; i <= l + 1; ++i) {
                 const R x = (R)(((double)(R)i - 0.5) / (double)(R)l);
-                R xj = zero, yj = zero, zj = zero, crho = uno;
-                if (mode == 1) {
+                R xj = zero, yj = zero, zj = zero, crho = uno, drho = zero;
+                if (mode == 1 || mode == 2) {
                     xj = u00 * std::sin(due * pi * x) * std::cos(due * pi * y) * std::cos(due * pi * z);
                     yj = -u00 * std::cos(due * pi * x) * std::sin(due * pi * y) * std::cos(due * pi * z);
-                    crho = uno + ((u00 * u00) / (qua * qua * tre)) * (std::cos(qua * pi * x) + std::cos(qua * pi * y)) *
-                                     (std::cos(qua * pi * z) + due);
+                    drho = ((u00 * u00) / (qua * qua * tre)) * (std::cos(qua * pi * x) + std::cos(qua * pi * y)) *
+                           (std::cos(qua * pi * z) + due);
+                    crho = uno + drho;
                 }
                 const R cvsq = xj * xj + yj * yj + zj * zj;
                 // c.u per direction in the reference's grouping (SRC/init.F90:159-177)
@@ -56,7 +60,7 @@ int init_feq_host(R** h, int l, int m, int n, int nx, int ny, int mode, R u00, i
                 const size_t id = (size_t)i + (size_t)nx * j + nxy * (size_t)(k - k0);
                 for (int d = 0; d < 19; ++d) {
                     const R c = rf * lin[d] + qf * (tre * sq[d] * sq[d] - cvsq);
-                    h[d][id] = crho * w[d] * (cte1 + c);
+                    h[d][id] = mode == 2 ? w[d] * (drho + crho * c) : crho * w[d] * (cte1 + c);
                 }
             }
         }

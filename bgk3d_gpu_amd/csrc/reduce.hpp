@@ -211,24 +211,29 @@ __global__ __launch_bounds__(kRedThreads) void draglift_partial(Lattice19<S> a, 
     }
 }
 
-// prof_k (SRC/prof_k.F90:47-79): line (ic, jc, 1..n), one thread per k.
-// out[4*(k-1) + {0,1,2,3}] = {u, w(y-vel), v(z-vel), den} in the storage kind.
+// Velocity/density along a grid line (SRC/prof_i.F90:47-78,
+// SRC/prof_j.F90:47-77, SRC/prof_k.F90:47-79) or at one point
+// (SRC/probe.F90:44-75; same sums, same divisions): axis 0 runs i at
+// (j, k) = (c1, c2), axis 1 runs j at (i, k), axis 2 runs k at (i, j), over
+// first .. first+count-1.  out[4*t + {0,1,2,3}] = u, w (y-velocity),
+// v (z-velocity), den, formed in the collision kind.
 template <typename S>
-__global__ void prof_k_kernel(Lattice19<S> a, Geometry g, int ic, int jc, typename Storage<S>::compute cte1,
-                              typename Storage<S>::compute* __restrict__ out)
+__global__ void prof_line_kernel(Lattice19<S> a, Geometry g, int axis, int c1, int c2, int first, int count,
+                                 typename Storage<S>::compute cte1, typename Storage<S>::compute* __restrict__ out)
 {
     typedef typename Storage<S>::compute R;
-    const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > g.n) return;
-    const long s = gidx(g, ic, jc, k);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int q = first + t;
+    const long s = axis == 0 ? gidx(g, q, c1, c2) : (axis == 1 ? gidx(g, c1, q, c2) : gidx(g, c1, c2, q));
     R x[19];
 #pragma unroll
     for (int d = 0; d < 19; ++d) x[d] = Storage<S>::load(a.p[d] + s);
     const R den = density_ref(x, cte1);
-    out[4 * (k - 1) + 0] = (x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) / den;
-    out[4 * (k - 1) + 1] = (x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]) / den;
-    out[4 * (k - 1) + 2] = (x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]) / den;
-    out[4 * (k - 1) + 3] = den;
+    out[4 * t + 0] = (x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) / den;
+    out[4 * t + 1] = (x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]) / den;
+    out[4 * t + 2] = (x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]) / den;
+    out[4 * t + 3] = den;
 }
 
 // Interior macroscopic fields (rho, ux, uy, uz) in FP64, i fastest; the

@@ -333,20 +333,46 @@ int draglift_gpu(real_t** a, const int* obs, int l, int m, int n, int nx, int ny
     return 0;
 }
 
-int prof_k_gpu(real_t** a, int l, int m, int n, int nx, int ny, int icoord, int jcoord, real_t cte1, real_t* out)
+// Line/point profile shared by prof_i/j/k_gpu and probe_gpu (axis 0: i at
+// (c1, c2) = (j, k); 1: j at (i, k); 2: k at (i, j)); 4 values per point.
+static int line_profile(real_t** a, int l, int m, int n, int nx, int ny, int axis, int c1, int c2, int first,
+                        int count, real_t cte1, real_t* out)
 {
-    if (icoord < 0 || icoord > l + 1 || jcoord < 0 || jcoord > m + 1 || n <= 0) return -1;
+    const int len = axis == 0 ? l : (axis == 1 ? m : n);
+    const int lim1 = axis == 0 ? m : l, lim2 = axis == 2 ? m : n;
+    if (l <= 0 || m <= 0 || n <= 0 || count <= 0 || first < 1 || first + count - 1 > len) return -1;
+    if (c1 < 0 || c1 > lim1 + 1 || c2 < 0 || c2 > lim2 + 1) return -1;
     double* host = nullptr;
-    const size_t bytes = (size_t)4 * n * sizeof(real_t);
+    const size_t bytes = (size_t)4 * count * sizeof(real_t);
     double* dev = scratch((bytes + sizeof(double) - 1) / sizeof(double), &host);
     if (!dev) return -2;
     real_t* d = reinterpret_cast<real_t*>(dev);
-    hipLaunchKernelGGL(prof_k_kernel<real_t>, dim3(ceil_div(n, 64)), dim3(64), 0, abi_stream(), lat_from(a),
-                       geom(l, m, n, nx, ny), icoord, jcoord, cte1, d);
-    report_launch("prof_k_kernel");
+    hipLaunchKernelGGL(prof_line_kernel<real_t>, dim3(ceil_div(count, 64)), dim3(64), 0, abi_stream(), lat_from(a),
+                       geom(l, m, n, nx, ny), axis, c1, c2, first, count, cte1, d);
+    report_launch("prof_line_kernel");
     BGK_HIP_CHECK(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, abi_stream()));
     BGK_HIP_CHECK(hipStreamSynchronize(abi_stream()));
     return 0;
+}
+
+int prof_i_gpu(real_t** a, int l, int m, int n, int nx, int ny, int jcoord, int kcoord, real_t cte1, real_t* out)
+{
+    return line_profile(a, l, m, n, nx, ny, 0, jcoord, kcoord, 1, l, cte1, out);
+}
+
+int prof_j_gpu(real_t** a, int l, int m, int n, int nx, int ny, int icoord, int kcoord, real_t cte1, real_t* out)
+{
+    return line_profile(a, l, m, n, nx, ny, 1, icoord, kcoord, 1, m, cte1, out);
+}
+
+int prof_k_gpu(real_t** a, int l, int m, int n, int nx, int ny, int icoord, int jcoord, real_t cte1, real_t* out)
+{
+    return line_profile(a, l, m, n, nx, ny, 2, icoord, jcoord, 1, n, cte1, out);
+}
+
+int probe_gpu(real_t** a, int l, int m, int n, int nx, int ny, int i0, int j0, int k0, real_t cte1, real_t* out)
+{
+    return line_profile(a, l, m, n, nx, ny, 0, j0, k0, i0, 1, cte1, out);
 }
 
 // Initial populations (host-side, as the reference's init), see host_init.hpp.

@@ -46,7 +46,8 @@ typedef struct bgk_config {
     int rounding;         /* 0 nearest-even, 1 stochastic (F16/BF16 stores only) */
     double svisc;         /* viscosity nu (namelist svisc); omega = 2/(6 nu + 1) */
     double u0;            /* lid velocity, or volume-force scale for the channel (namelist u0) */
-    int init;             /* 0 rest, 1 3-D Taylor-Green (SRC/init.F90) */
+    int init;             /* 0 rest, 1 3-D Taylor-Green (SRC/init.F90), 2 the same with
+                             the density perturbation kept (not in the reference) */
     int obstacle;         /* 0 none, 1 hollow sphere shell (SRC/build.obs.F90), 2 solid sphere */
     int obs_ic, obs_jc, obs_kc, obs_radius; /* sphere centre/radius; < 0 = reference defaults */
     double u_inflow;      /* inflow velocity; < 0 = reference value (default-real 0.1) */
@@ -88,6 +89,37 @@ int bgk_fields(bgk_handle* h, double* rho, double* ux, double* uy, double* uz);
  * prof_k prints it (SRC/prof_k.F90:47-79): out[4*(k-1)+{0,1,2,3}] =
  * {u, w (y-velocity), v (z-velocity), rho}, computed in the collision kind. */
 int bgk_prof_k(bgk_handle* h, int icoord, int jcoord, double* out);
+
+/* The same along any axis: axis 0 runs i at (j, k) = (c1, c2)
+ * (SRC/prof_i.F90:47-78), axis 1 runs j at (i, k) (SRC/prof_j.F90:47-77),
+ * axis 2 runs k at (i, j) (SRC/prof_k.F90); 4 values per point. */
+int bgk_profile(bgk_handle* h, int axis, int c1, int c2, double* out);
+
+/* One cell: {u, v, w, rho} (x, y, z velocity, density) as the reference's
+ * probe prints them (SRC/probe.F90:44-75). */
+int bgk_probe(bgk_handle* h, int i0, int j0, int k0, double out[4]);
+
+/* Restart file in the reference's format (SRC/save_raw.F90:27-69): Fortran
+ * sequential unformatted records (4-byte length markers), the time step,
+ * then a01..a19 over (0:l+1, 0:m+1, 0:n+1), i fastest, in the storage type.
+ * The halo cells hold what the handle holds there (the library never reads
+ * them back: boundaries are applied in the fused step). */
+int bgk_save_raw(bgk_handle* h, const char* path, int itime);
+
+/* Read such a file (SRC/restore_raw.F90:27-64): the grid and storage type
+ * must match; the handle's step count becomes the file's time step. */
+int bgk_restore_raw(bgk_handle* h, const char* path, int* itime);
+
+/* Velocity file (SRC/save_vel.F90): a record with the time step, then one
+ * record of (u, v, w) per interior cell, i fastest, in the collision kind. */
+int bgk_save_vel(bgk_handle* h, const char* path, int itime);
+
+/* Legacy VTK file with an ASCII header and big-endian FP32 vectors, as the
+ * reference writes them: kind 0 velocity on the xy plane k = index
+ * (SRC/vtk_xy.binary.F90), 1 xz plane j = index (SRC/vtk_xz.binary.F90),
+ * 2 yz plane i = index (SRC/vtk_yz.binary.F90), 3 vorticity on the xy plane
+ * k = index (SRC/vtk_om.xy.binary.F90; the reference's sign convention). */
+int bgk_write_vtk(bgk_handle* h, int kind, int index, const char* path);
 
 /* Interior values of population d (0..18, direction d+1), as double. */
 int bgk_get_population(bgk_handle* h, int d, double* out);

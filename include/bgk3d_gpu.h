@@ -133,8 +133,25 @@ int draglift_gpu(bgk_real_t** a, const int* obs, int l, int m, int n, int nx, in
 int prof_k_gpu(bgk_real_t** a, int l, int m, int n, int nx, int ny, int icoord, int jcoord, bgk_real_t cte1,
                bgk_real_t* out);
 
+/* The same along i at (jcoord, kcoord), 4*l values.
+ * Replaces SRC/prof_i.F90:47-78. */
+int prof_i_gpu(bgk_real_t** a, int l, int m, int n, int nx, int ny, int jcoord, int kcoord, bgk_real_t cte1,
+               bgk_real_t* out);
+
+/* The same along j at (icoord, kcoord), 4*m values.
+ * Replaces SRC/prof_j.F90:47-77. */
+int prof_j_gpu(bgk_real_t** a, int l, int m, int n, int nx, int ny, int icoord, int kcoord, bgk_real_t cte1,
+               bgk_real_t* out);
+
+/* One cell: out = {u, v, w, rho} (x, y, z velocity and density), the values
+ * the reference's probe prints.  Replaces SRC/probe.F90:44-75. */
+int probe_gpu(bgk_real_t** a, int l, int m, int n, int nx, int ny, int i0, int j0, int k0, bgk_real_t cte1,
+              bgk_real_t* out);
+
 /* Initial state f = f_eq into HOST arrays h[0..18] (all cells incl. halo),
- * shifted storage: mode 0 rest, mode 1 3-D Taylor-Green with amplitude u00.
+ * shifted storage: mode 0 rest, mode 1 3-D Taylor-Green with amplitude u00,
+ * mode 2 the same with the density perturbation kept (the reference's shifted
+ * form drops it; no reference counterpart).
  * Host-side like the reference's init, bit-identical to it.
  * Replaces SRC/init.F90:31-209 (PERIODIC + TGV3D branch for mode 1). */
 int bgk_init_host(bgk_real_t** h, int l, int m, int n, int nx, int ny, int mode, bgk_real_t u00);

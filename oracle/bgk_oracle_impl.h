@@ -65,6 +65,9 @@ void FN(oracle_hencol_)(R svisc, R u0, int m, R *omega, R *fgrad)
 /* init: SRC/init.F90:31-209, shifted storage (cte1 = 0 there, :71-75).      */
 /* mode 0: rest flow (all populations 0).                                    */
 /* mode 1: 3-D Taylor-Green (PERIODIC + TGV3D, :114-148).                     */
+/* mode 2: mode 1 with the density perturbation kept (the reference's        */
+/*         shifted form stores crho*w*c and drops w*(crho-1); no reference   */
+/*         counterpart, pinned by its moments in the tests).                 */
 /* Every cell including the halo (0..l1, 0..m1, 0..n1) is written; the       */
 /* ipad/jpad padding is left untouched (it keeps the huge() poison).         */
 /* ------------------------------------------------------------------------ */
@@ -85,17 +88,18 @@ void FN(oracle_init_)(R **a, int l, int m, int n, int nx, int ny, int mode, R u0
             const R y = (R)(((double)(R)j - 0.5) / (double)(R)m);
             for (int i = 0; i <= l + 1; ++i) {
                 const R x = (R)(((double)(R)i - 0.5) / (double)(R)l);
-                R xj = zero, yj = zero, zj = zero, crho = uno;
-                if (mode == 1) {
+                R xj = zero, yj = zero, zj = zero, crho = uno, drho = zero;
+                if (mode == 1 || mode == 2) {
                     /* SRC/init.F90:142-148 */
                     xj = u00 * CAT(SIN, SFX)(due * pi * x) * CAT(COS, SFX)(due * pi * y) *
                          CAT(COS, SFX)(due * pi * z);
                     yj = -u00 * CAT(COS, SFX)(due * pi * x) * CAT(SIN, SFX)(due * pi * y) *
                          CAT(COS, SFX)(due * pi * z);
                     zj = zero;
-                    crho = uno + ((u00 * u00) / (qua * qua * tre)) *
-                                     (CAT(COS, SFX)(qua * pi * x) + CAT(COS, SFX)(qua * pi * y)) *
-                                     (CAT(COS, SFX)(qua * pi * z) + due);
+                    drho = ((u00 * u00) / (qua * qua * tre)) *
+                           (CAT(COS, SFX)(qua * pi * x) + CAT(COS, SFX)(qua * pi * y)) *
+                           (CAT(COS, SFX)(qua * pi * z) + due);
+                    crho = uno + drho;
                 }
                 const R cvsq = xj * xj + yj * yj + zj * zj;
                 /* SRC/init.F90:159-177 */
@@ -124,8 +128,10 @@ void FN(oracle_init_)(R **a, int l, int m, int n, int nx, int ny, int mode, R u0
                 const size_t id = IDX(i, j, k);
                 for (int d = 0; d < 19; ++d) {
                     const R w = wclass[d] == 0 ? p0 : (wclass[d] == 1 ? p1 : p2);
-                    /* SRC/init.F90:179-197: crho*pX*(cte1+cxNN) */
-                    a[d][id] = crho * w * (cte1 + c[d]);
+                    /* SRC/init.F90:179-197: crho*pX*(cte1+cxNN).  Mode 2 keeps
+                     * the density perturbation the shifted form drops:
+                     * f - w = w (rho - 1) + w rho c = w (drho + crho c). */
+                    a[d][id] = mode == 2 ? w * (drho + crho * c[d]) : crho * w * (cte1 + c[d]);
                 }
             }
         }
@@ -598,33 +604,53 @@ void FN(oracle_varm_)(R **a, int l, int m, int n, int nx, int ny, R *u_k, R *w_k
     for (int j = 0; j < m; ++j) den_j[j] = den_j[j] * rvol;
 }
 
-/* prof_k: SRC/prof_k.F90:47-79, the line (icoord, jcoord, 1..n).
- * out[4*(k-1) + {0,1,2,3}] = u, w (y-velocity), v (z-velocity), den. */
-void FN(oracle_prof_k_)(R **a, int l, int m, int n, int nx, int ny, int ic, int jc, R *out)
+/* Line profiles (SRC/prof_i.F90:47-78, SRC/prof_j.F90:47-77,
+ * SRC/prof_k.F90:47-79) and the point probe (SRC/probe.F90:44-75): the
+ * three routines form, per cell, den = a01+...+a19+cte1 and the momentum
+ * sums in the same order and divide each by den; probe does the same at one
+ * point (xj/rho, yj/rho, zj/rho, rho).  Line along axis 0 (i = 1..l at
+ * (c1, c2) = (j, k)), 1 (j = 1..m at (i, k)) or 2 (k = 1..n at (i, j)).
+ * out[4*(t-1) + {0,1,2,3}] = u, w (y-velocity), v (z-velocity), den. */
+static void FN(line_point_)(R **a, size_t s, R *o)
+{
+    const R uno = (R)1, cte1 = uno;
+    const R den = +A(1)[s] + A(2)[s] + A(3)[s] + A(4)[s] + A(5)[s] + A(6)[s] + A(7)[s] + A(8)[s] + A(9)[s] +
+                  A(10)[s] + A(11)[s] + A(12)[s] + A(13)[s] + A(14)[s] + A(15)[s] + A(16)[s] + A(17)[s] +
+                  A(18)[s] + A(19)[s] + cte1;
+    const R u = +(A(1)[s] + A(2)[s] + A(3)[s] + A(4)[s] + A(5)[s] - A(10)[s] - A(11)[s] - A(12)[s] -
+                  A(13)[s] - A(14)[s]) /
+                den;
+    const R w = +(A(3)[s] + A(7)[s] + A(8)[s] + A(9)[s] + A(12)[s] - A(1)[s] - A(10)[s] - A(16)[s] -
+                  A(17)[s] - A(18)[s]) /
+                den;
+    const R v = +(A(4)[s] + A(6)[s] + A(7)[s] + A(13)[s] + A(18)[s] - A(2)[s] - A(9)[s] - A(11)[s] -
+                  A(15)[s] - A(16)[s]) /
+                den;
+    o[0] = u;
+    o[1] = w;
+    o[2] = v;
+    o[3] = den;
+}
+
+void FN(oracle_prof_line_)(R **a, int l, int m, int n, int nx, int ny, int axis, int c1, int c2, R *out)
 {
     const size_t nxy = (size_t)nx * (size_t)ny;
-    const R uno = (R)1, cte1 = uno;
-    (void)l;
-    (void)m;
-    for (int k = 1; k <= n; ++k) {
-        const size_t s = IDX(ic, jc, k);
-        const R den = +A(1)[s] + A(2)[s] + A(3)[s] + A(4)[s] + A(5)[s] + A(6)[s] + A(7)[s] + A(8)[s] + A(9)[s] +
-                      A(10)[s] + A(11)[s] + A(12)[s] + A(13)[s] + A(14)[s] + A(15)[s] + A(16)[s] + A(17)[s] +
-                      A(18)[s] + A(19)[s] + cte1;
-        const R u = +(A(1)[s] + A(2)[s] + A(3)[s] + A(4)[s] + A(5)[s] - A(10)[s] - A(11)[s] - A(12)[s] -
-                      A(13)[s] - A(14)[s]) /
-                    den;
-        const R w = +(A(3)[s] + A(7)[s] + A(8)[s] + A(9)[s] + A(12)[s] - A(1)[s] - A(10)[s] - A(16)[s] -
-                      A(17)[s] - A(18)[s]) /
-                    den;
-        const R v = +(A(4)[s] + A(6)[s] + A(7)[s] + A(13)[s] + A(18)[s] - A(2)[s] - A(9)[s] - A(11)[s] -
-                      A(15)[s] - A(16)[s]) /
-                    den;
-        out[4 * (k - 1) + 0] = u;
-        out[4 * (k - 1) + 1] = w;
-        out[4 * (k - 1) + 2] = v;
-        out[4 * (k - 1) + 3] = den;
+    const int len = axis == 0 ? l : (axis == 1 ? m : n);
+    for (int t = 1; t <= len; ++t) {
+        const size_t s = axis == 0 ? IDX(t, c1, c2) : (axis == 1 ? IDX(c1, t, c2) : IDX(c1, c2, t));
+        FN(line_point_)(a, s, out + 4 * (t - 1));
     }
+}
+
+void FN(oracle_prof_k_)(R **a, int l, int m, int n, int nx, int ny, int ic, int jc, R *out)
+{
+    FN(oracle_prof_line_)(a, l, m, n, nx, ny, 2, ic, jc, out);
+}
+
+void FN(oracle_probe_)(R **a, int nx, int ny, int i0, int j0, int k0, R *out)
+{
+    const size_t nxy = (size_t)nx * (size_t)ny;
+    FN(line_point_)(a, IDX(i0, j0, k0), out);
 }
 
 /* draglift: SRC/draglift.F90:48-179.  Momentum exchange over fluid cells of

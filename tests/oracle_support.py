@@ -199,6 +199,23 @@ def prof_k(A, l, m, n, ic, jc, prec="sp"):
     return out
 
 
+def prof_line(A, l, m, n, axis, c1, c2, prec="sp"):
+    """Profile along axis 0 (i at (j, k) = (c1, c2)), 1 (j at (i, k)) or 2 (k
+    at (i, j)): rows of (u, w, v, den) (SRC/prof_i.F90, prof_j.F90, prof_k.F90)."""
+    nx, ny = A.shape[3], A.shape[2]
+    out = np.zeros(((l, m, n)[axis], 4), np_dtype(prec))
+    fn("prof_line", prec)(ptrs(A), l, m, n, nx, ny, axis, c1, c2, out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+def probe(A, i0, j0, k0, prec="sp"):
+    """(u, v, w, rho) at one cell (SRC/probe.F90)."""
+    nx, ny = A.shape[3], A.shape[2]
+    out = np.zeros(4, np_dtype(prec))
+    fn("probe", prec)(ptrs(A), nx, ny, i0, j0, k0, out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
 def draglift(A, obs, l, m, n, radius, u_inflow=U_INFLOW_REF, prec="sp"):
     nx, ny = A.shape[3], A.shape[2]
     ct = ctype(prec)
@@ -219,8 +236,13 @@ def ref_available(case="ldc", prec="sp", omp=False) -> bool:
 
 
 def run_ref(case, l, m, n, steps, prec="sp", svisc=0.064, u0=0.1, icheck=None, init_v=0, ipad=0, jpad=0,
-            mode=0, A_in=None, omp=False, threads=None, keep_dir=None):
-    """Run the reference harness; returns (A_out, outdir_texts, stdout)."""
+            mode=0, A_in=None, omp=False, threads=None, keep_dir=None, write=False, restore=None):
+    """Run the reference harness; returns (A_out, outdir_texts, stdout).
+
+    write=True adds texts["files"] = {name: bytes} of what the reference's
+    save_raw, save_vel and VTK writers produced at the last step;
+    restore=<bytes> starts from that restart file (the reference's
+    restore_raw) instead of init."""
     exe = REF_DIR / f"{case}_{prec}{'_omp' if omp else ''}" / "ref_harness.x"
     d = Path(keep_dir) if keep_dir else Path(tempfile.mkdtemp(prefix="bgkref_"))
     d.mkdir(parents=True, exist_ok=True)
@@ -228,18 +250,25 @@ def run_ref(case, l, m, n, steps, prec="sp", svisc=0.064, u0=0.1, icheck=None, i
         if A_in is not None:
             init_v = -1
             np.ascontiguousarray(A_in).tofile(d / "in.bin")
+        if restore is not None:
+            init_v = -2
+            (d / "restore.bin").write_bytes(restore)
         env = dict(os.environ)
         if threads:
             env["OMP_NUM_THREADS"] = str(threads)
         cmd = [str(exe), str(d), str(l), str(m), str(n), repr(float(svisc)), repr(float(u0)), str(steps),
-               str(icheck or max(steps, 1)), str(init_v), str(ipad), str(jpad), str(mode)]
-        out = subprocess.run(cmd, check=True, capture_output=True, text=True, env=env).stdout
+               str(icheck or max(steps, 1)), str(init_v), str(ipad), str(jpad), str(mode), "1" if write else "0"]
+        out = subprocess.run(cmd, check=True, capture_output=True, text=True, env=env, cwd=d).stdout
         nx, ny, nz = l + 2 + ipad, m + 2 + jpad, n + 2
         A = np.fromfile(d / "out.bin", dtype=np_dtype(prec)).reshape(19, nz, ny, nx)
         texts = {}
-        for name in ("diagno.dat", "prof_k.dat", "u_med.dat", "drag.lift.dat"):
+        for name in ("diagno.dat", "prof_i.dat", "prof_j.dat", "prof_k.dat", "probe.dat", "u_med.dat",
+                     "drag.lift.dat"):
             p = d / name
             texts[name] = p.read_text() if p.exists() else ""
+        if write:
+            texts["files"] = {p.name: p.read_bytes() for p in sorted(d.iterdir())
+                              if p.name.startswith(("save.", "save_vel.", "tec_"))}
         if (d / "obs.bin").exists():
             texts["obs"] = np.fromfile(d / "obs.bin", dtype=np.int32).reshape(n, m, l)
         return A, texts, out
