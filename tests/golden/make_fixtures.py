@@ -6,8 +6,15 @@ by oracle/Makefile, linked with our small harness) and records, per run:
   * sha256 of the interior populations (fluid cells only when there is an
     obstacle), C order (k, j, i), native little-endian bytes;
   * per-population sums in float64 (to locate a mismatch);
-  * the reference's own text outputs (diagno.dat, prof_k.dat, u_med.dat,
-    drag.lift.dat) as written by its diagnostics;
+  * the reference's own text outputs (diagno.dat, probe.dat, prof_i.dat,
+    prof_j.dat, prof_k.dat, u_med.dat, drag.lift.dat) as written by its
+    diagnostics;
+  * digests of the files its writers produce at the last step: the restart
+    file of save_raw (time step and the interior of every record -- the halo
+    cells hold stale boundary values nobody reads), save_vel, and the VTK
+    planes vtk_xy/xz/yz_bin and vtk_om_xy_bin (whole files);
+  * a restart pin: the reference's own save_raw at half the steps, restored
+    by its restore_raw and run on, ends on the full run's state;
   * for the single-routine pins: a seeded random state pushed through one
     `boundaries` call or one `col_MC` call.
 
@@ -25,6 +32,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE.parent))
+import format_support as F  # noqa: E402
 import oracle_support as O  # noqa: E402
 
 # (name, case, l, m, n, steps, svisc, u0, init_v, icheck)
@@ -66,22 +74,39 @@ def random_state(l, m, n, prec, seed):
 
 
 def main():
-    out = {"generator": "tests/golden/make_fixtures.py", "runs": {}, "unit": {}}
+    out = {"generator": "tests/golden/make_fixtures.py", "runs": {}, "unit": {}, "restart": {}}
     for name, case, l, m, n, steps, svisc, u0, init_v, icheck in RUNS:
         for prec in ("sp", "dp"):
             if not O.ref_available(case, prec):
                 raise SystemExit(f"oracle/_ref/{case}_{prec} not built")
             A, texts, stdout = O.run_ref(case, l, m, n, steps, prec=prec, svisc=svisc, u0=u0, icheck=icheck,
-                                         init_v=init_v)
+                                         init_v=init_v, write=True)
             obs = texts.get("obs")
             I = interior_bytes(A, obs)
             out["runs"][f"{name}_{prec}"] = {
                 "case": case, "l": l, "m": m, "n": n, "steps": steps, "svisc": svisc, "u0": u0, "init": init_v,
                 "icheck": icheck, "prec": prec, "sha256_interior": digest(I),
                 "pop_sums": [float(x) for x in I.reshape(19, -1).astype(np.float64).sum(axis=1)],
-                "texts": {k: v for k, v in texts.items() if k != "obs"},
+                "texts": {k: v for k, v in texts.items() if k not in ("obs", "files")},
+                "files": F.file_digests(texts["files"], l, m, n, prec),
             }
             print(name, prec, "done", flush=True)
+    # restart pin: save_raw at half the steps, restore_raw, run on
+    for name, case, l, m, n, steps, svisc, u0, init_v, icheck in RUNS[:1] + RUNS[2:3]:
+        for prec in ("sp", "dp"):
+            half = steps // 2
+            _, texts, _ = O.run_ref(case, l, m, n, half, prec=prec, svisc=svisc, u0=u0, init_v=init_v, write=True)
+            raw = texts["files"][f"save.{half:08d}.bin"]
+            A, _, _ = O.run_ref(case, l, m, n, steps, prec=prec, svisc=svisc, u0=u0, restore=raw)
+            full = out["runs"][f"{name}_{prec}"]
+            ok = digest(interior_bytes(A)) == full["sha256_interior"]
+            if not ok:
+                raise SystemExit(f"restart of {name} {prec} does not reproduce the full run")
+            out["restart"][f"{name}_{prec}"] = {
+                "run": f"{name}_{prec}", "save_step": half,
+                "save_raw": F.file_digests({f"save.{half:08d}.bin": raw}, l, m, n, prec)["save_raw"],
+            }
+            print("restart", name, prec, "done", flush=True)
     for name, case, l, m, n, mode in UNIT:
         for prec in ("sp", "dp"):
             seed = int(hashlib.sha256(f"{name}-{prec}".encode()).hexdigest()[:8], 16)
