@@ -17,6 +17,11 @@
 //
 // Steps run in chunks of at most `chunk_steps`, shortened so that no request
 // overshoots its step count; requests join and leave only between chunks.
+// Policies: every admitted request advances in every chunk (processor
+// sharing, arrival-order admission), or only the admitted requests with the
+// least work left, as many as keep a given number of cells per step in
+// flight (shortest remaining work first, which cuts waiting for the small
+// requests of a mix while the GPU stays saturated).
 #include "engine_internal.hpp"
 
 #include <algorithm>
@@ -163,6 +168,8 @@ struct bgks_server {
     std::deque<long long> queue;
     std::map<long long, std::unique_ptr<Instance>> all;
     long long next_id = 1;
+    int policy = BGKS_ALL_ACTIVE;
+    long long min_cells = 0;  // BGKS_SHORTEST_FIRST: cells advanced per step at least
     // per storage group: device tables and pinned staging
     struct Group {
         void* d_slots = nullptr;
@@ -258,20 +265,47 @@ int by_group(int gid, F&& f)
     return fail(BGK_EINVAL, "storage group %d", gid);
 }
 
+long long remaining_work(const Instance& in)
+{
+    return (long long)in.h.cfg.l * in.h.cfg.m * in.h.cfg.n * (in.req.steps - in.h.steps);
+}
+
+int start(bgks_server& s, Instance& in, size_t off)
+{
+    in.off = off;
+    in.has_mem = true;
+    void* region = static_cast<char*>(s.arena_mem) + off;
+    place_lattices(in.h, region);
+    ENG_HIP(hipMemsetAsync(region, 0, in.len, s.stream));  // rest state: shifted populations are zero
+    in.state = BGKS_ACTIVE;
+    in.t_admit = clock_now();
+    return 0;
+}
+
+// Arrival order: the head of the queue waits for room.  Shortest first:
+// queued requests in order of their work, each admitted if it fits.
 int admit(bgks_server& s)
 {
-    while (!s.queue.empty()) {
-        Instance& in = *s.all.at(s.queue.front());
+    if (s.policy == BGKS_ALL_ACTIVE) {
+        while (!s.queue.empty()) {
+            Instance& in = *s.all.at(s.queue.front());
+            size_t off = 0;
+            if (!s.arena.alloc(in.len, off)) break;
+            if (int rc = start(s, in, off)) return rc;
+            s.queue.pop_front();
+        }
+        return 0;
+    }
+    std::vector<long long> order(s.queue.begin(), s.queue.end());
+    std::stable_sort(order.begin(), order.end(), [&](long long a, long long b) {
+        return remaining_work(*s.all.at(a)) < remaining_work(*s.all.at(b));
+    });
+    for (long long id : order) {
+        Instance& in = *s.all.at(id);
         size_t off = 0;
-        if (!s.arena.alloc(in.len, off)) break;  // arrival order: the head waits for room
-        in.off = off;
-        in.has_mem = true;
-        void* region = static_cast<char*>(s.arena_mem) + off;
-        place_lattices(in.h, region);
-        ENG_HIP(hipMemsetAsync(region, 0, in.len, s.stream));  // rest state: shifted populations are zero
-        in.state = BGKS_ACTIVE;
-        in.t_admit = clock_now();
-        s.queue.pop_front();
+        if (!s.arena.alloc(in.len, off)) continue;
+        if (int rc = start(s, in, off)) return rc;
+        s.queue.erase(std::find(s.queue.begin(), s.queue.end(), id));
     }
     return 0;
 }
@@ -329,6 +363,16 @@ int bgks_create(int device, size_t budget_bytes, int chunk_steps, bgks_server** 
     return 0;
 }
 
+int bgks_set_policy(bgks_server* s, int policy, long long min_cells_per_step)
+{
+    if (!s) return fail(BGK_EINVAL, "null server");
+    if (policy != BGKS_ALL_ACTIVE && policy != BGKS_SHORTEST_FIRST) return fail(BGK_EINVAL, "policy %d", policy);
+    if (min_cells_per_step < 0) return fail(BGK_EINVAL, "min_cells_per_step %lld", min_cells_per_step);
+    s->policy = policy;
+    s->min_cells = min_cells_per_step;
+    return 0;
+}
+
 int bgks_destroy(bgks_server* s)
 {
     if (!s) return 0;
@@ -381,6 +425,20 @@ int bgks_advance(bgks_server* s, long long* done_ids, int max_done, int* n_done)
     std::vector<Instance*> active;
     for (auto& kv : s->all)
         if (kv.second->state == BGKS_ACTIVE) active.push_back(kv.second.get());
+    if (s->policy == BGKS_SHORTEST_FIRST && !active.empty()) {
+        // the admitted requests with the least work left, enough of them to
+        // keep at least min_cells cells per step in flight
+        std::stable_sort(active.begin(), active.end(), [](const Instance* a, const Instance* b) {
+            return remaining_work(*a) < remaining_work(*b);
+        });
+        long long cells = 0;
+        size_t take = 0;
+        while (take < active.size() && (take == 0 || cells < s->min_cells)) {
+            const Instance* in = active[take++];
+            cells += (long long)in->h.cfg.l * in->h.cfg.m * in->h.cfg.n;
+        }
+        active.resize(take);
+    }
     long long K = s->chunk;
     for (Instance* in : active) K = std::min(K, in->req.steps - in->h.steps);
     K = std::max(K, 0LL);

@@ -88,26 +88,20 @@ struct Rounder<f16_t, 0> {
 };
 template <>
 struct Rounder<f16_t, 1> {
-    // Unbiased stochastic rounding to FP16 from the FP32 bits: in the FP16
-    // normal range the 13 mantissa bits FP16 drops receive a uniform 13-bit
-    // addend and are truncated (a carry rounds up with probability equal to
-    // the dropped fraction); below it the spacing is the fixed 2^-24, so
-    // |v|*2^24 plus a uniform [0,1) is floored.  Both results are exact FP16
-    // values, so the final conversion does not round again.  Uses the top
-    // bits of r.
+    // Unbiased stochastic rounding to FP16 without branches: add a uniform
+    // fraction of the FP16 spacing at |v| (2^(e-10) in the normal range,
+    // 2^-24 below it; 13 random bits, so the sum is exact in FP32 for the
+    // normal range) away from zero, then convert with round-toward-zero
+    // (v_cvt_pkrtz_f16_f32), which truncates to the spacing in the normal
+    // and the subnormal range alike.  |v| >= 2^15 converts to nearest.
     __device__ __forceinline__ static f16_t cvt(float v, uint32_t r)
     {
-        const uint32_t u = __float_as_uint(v);
-        const uint32_t sign = u & 0x80000000u;
-        uint32_t a = u ^ sign;
-        if (a >= 0x38800000u) {                        // |v| >= 2^-14
-            if (a >= 0x477fe000u) return (f16_t)v;     // at the top of the range: nearest
-            a = (a + (r >> 19)) & ~0x1fffu;
-            return (f16_t)__uint_as_float(a | sign);
-        }
-        const float s = __uint_as_float(a) * 16777216.0f;  // exact
-        const float t = __builtin_floorf(s + (float)(r >> 8) * (1.0f / 16777216.0f));
-        return (f16_t)__uint_as_float(__float_as_uint(t * 5.9604644775390625e-8f) | sign);
+        const uint32_t e = __float_as_uint(v) & 0x7f800000u;
+        const float ulp = __uint_as_float((e > 0x38800000u ? e : 0x38800000u) - 0x05000000u);
+        const float du = (float)(r >> 19) * 1.220703125e-4f * ulp;
+        const float t = v + __builtin_copysignf(du, v);
+        const f16_t sr = __builtin_amdgcn_cvt_pkrtz(t, t)[0];
+        return e >= 0x47000000u ? (f16_t)v : sr;
     }
 };
 template <>

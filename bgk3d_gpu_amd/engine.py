@@ -43,6 +43,12 @@ def lib() -> ctypes.CDLL:
         L.bgk_fields.argtypes = [vp, vp, vp, vp, vp]
         L.bgk_get_population.argtypes = [vp, ci, vp]
         L.bgk_prof_k.argtypes = [vp, ci, ci, vp]
+        L.bgk_profile.argtypes = [vp, ci, ci, ci, vp]
+        L.bgk_probe.argtypes = [vp, ci, ci, ci, vp]
+        L.bgk_save_raw.argtypes = [vp, ctypes.c_char_p, ci]
+        L.bgk_restore_raw.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ci)]
+        L.bgk_save_vel.argtypes = [vp, ctypes.c_char_p, ci]
+        L.bgk_write_vtk.argtypes = [vp, ci, ci, ctypes.c_char_p]
         L.bgk_set_population.argtypes = [vp, ci, vp]
         L.bgk_info.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
         L.bgk_last_error.restype = ctypes.c_char_p
@@ -122,6 +128,35 @@ class Engine:
         _check(self.L.bgk_prof_k(self.h, c.l // 2 if ic is None else ic, c.m // 2 if jc is None else jc,
                                  out.ctypes.data), "bgk_prof_k")
         return out
+
+    def profile(self, axis: int, c1: int, c2: int) -> np.ndarray:
+        """(len, 4) rows of u, w (y-velocity), v (z-velocity), rho along axis
+        0 (i at (j, k) = (c1, c2)), 1 (j at (i, k)) or 2 (k at (i, j))."""
+        c = self.case
+        out = np.zeros(((c.l, c.m, c.n)[axis], 4))
+        _check(self.L.bgk_profile(self.h, axis, c1, c2, out.ctypes.data), "bgk_profile")
+        return out
+
+    def probe(self, i: int, j: int, k: int) -> np.ndarray:
+        """u, v, w, rho at one cell."""
+        out = np.zeros(4)
+        _check(self.L.bgk_probe(self.h, i, j, k, out.ctypes.data), "bgk_probe")
+        return out
+
+    def save_raw(self, path, itime: int):
+        _check(self.L.bgk_save_raw(self.h, str(path).encode(), itime), "bgk_save_raw")
+
+    def restore_raw(self, path) -> int:
+        it = ctypes.c_int()
+        _check(self.L.bgk_restore_raw(self.h, str(path).encode(), ctypes.byref(it)), "bgk_restore_raw")
+        return it.value
+
+    def save_vel(self, path, itime: int):
+        _check(self.L.bgk_save_vel(self.h, str(path).encode(), itime), "bgk_save_vel")
+
+    def write_vtk(self, kind: int, index: int, path):
+        """kind 0 xy, 1 xz, 2 yz velocity plane; 3 xy vorticity plane."""
+        _check(self.L.bgk_write_vtk(self.h, kind, index, str(path).encode()), "bgk_write_vtk")
 
     def population(self, d: int) -> np.ndarray:
         c = self.case

@@ -61,6 +61,9 @@ def v1(prec: str = "sp") -> ctypes.CDLL:
         lib.varm_gpu.argtypes = [vp] + [ci] * 5 + [R, vp, vp, vp, vp]
         lib.draglift_gpu.argtypes = [vp, vp] + [ci] * 9 + [R, vp]
         lib.prof_k_gpu.argtypes = [vp] + [ci] * 7 + [R, vp]
+        lib.prof_i_gpu.argtypes = [vp] + [ci] * 7 + [R, vp]
+        lib.prof_j_gpu.argtypes = [vp] + [ci] * 7 + [R, vp]
+        lib.probe_gpu.argtypes = [vp] + [ci] * 8 + [R, vp]
         lib.bgk_set_stream.argtypes = [vp]
         lib.bgk_dev_malloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
         lib.bgk_dev_free.argtypes = [vp]

@@ -28,6 +28,7 @@ from . import engine as E
 from .cases import Case
 
 STATE = {0: "queued", 1: "active", 2: "done"}
+POLICY = {"all": 0, "shortest": 1}
 
 
 class BGKSRequest(ctypes.Structure):
@@ -43,6 +44,7 @@ def lib() -> ctypes.CDLL:
         rq = ctypes.POINTER(BGKSRequest)
         L.bgks_create.argtypes = [ci, ctypes.c_size_t, ci, ctypes.POINTER(vp)]
         L.bgks_destroy.argtypes = [vp]
+        L.bgks_set_policy.argtypes = [vp, ci, ll]
         L.bgks_request_bytes.argtypes = [rq, ctypes.POINTER(ctypes.c_size_t)]
         L.bgks_submit.argtypes = [vp, rq, ctypes.POINTER(ll)]
         L.bgks_advance.argtypes = [vp, ctypes.POINTER(ll), ci, ctypes.POINTER(ci)]
@@ -120,11 +122,18 @@ def make_workload(count: int, steps: int = 5000, rate: float | None = None, size
 
 
 class Server:
-    def __init__(self, budget_bytes: int, chunk_steps: int = 100, device: int = 0):
+    """policy 'all': every admitted request advances each chunk, arrival-order
+    admission; 'shortest': least remaining work first, enough requests to keep
+    `min_cells` cells per step in flight."""
+
+    def __init__(self, budget_bytes: int, chunk_steps: int = 100, device: int = 0, policy: str = "all",
+                 min_cells: int = 0):
         self.L = lib()
         h = ctypes.c_void_p()
         E._check(self.L.bgks_create(device, budget_bytes, chunk_steps, ctypes.byref(h)), "bgks_create")
         self.h = h.value
+        if policy != "all" or min_cells:
+            E._check(self.L.bgks_set_policy(self.h, POLICY[policy], min_cells), "bgks_set_policy")
 
     def submit(self, r: Request) -> int:
         rid = ctypes.c_longlong()
@@ -208,18 +217,18 @@ def _wait_until(t):
 
 
 def run_batched(requests, budget_bytes: int, chunk_steps: int = 100, online: bool = False,
-                progress=None) -> dict:
+                progress=None, policy: str = "all", min_cells: int = 0) -> dict:
     """Serve `requests` with the batching server.  Offline: all submitted at
     the start.  Online: each submitted at its arrival time (checked between
     chunks, so a request waits at most one chunk to be queued)."""
     order = sorted(range(len(requests)), key=lambda q: requests[q].arrival)
     records = [None] * len(requests)
-    with Server(budget_bytes, chunk_steps) as srv:
+    with Server(budget_bytes, chunk_steps, policy=policy, min_cells=min_cells) as srv:
         ids = {}
         nxt = 0
         t0 = clock()
         finished = 0
-        peak_active = 0
+        peak_admitted = 0
         while finished < len(requests):
             now = clock()
             while nxt < len(order) and (not online or t0 + requests[order[nxt]].arrival <= now):
@@ -233,7 +242,7 @@ def run_batched(requests, budget_bytes: int, chunk_steps: int = 100, online: boo
                 _wait_until(t0 + requests[order[nxt]].arrival)
                 continue
             done = srv.advance()
-            peak_active = max(peak_active, srv.stats()["active"] + len(done))
+            peak_admitted = max(peak_admitted, srv.stats()["active"] + len(done))
             for rid in done:
                 q = ids[rid]
                 records[q] = srv.result(rid)
@@ -244,9 +253,9 @@ def run_batched(requests, budget_bytes: int, chunk_steps: int = 100, online: boo
                 progress(f"batched: {finished}/{len(requests)} done")
         t_end = clock()
         st = srv.stats()
-    out = _summary(requests, records, t0, t_end, "batched")
+    out = _summary(requests, records, t0, t_end, f"batched-{policy}")
     out.update(budget_bytes=budget_bytes, chunk_steps=chunk_steps, launches=st["launches"], chunks=st["chunks"],
-               peak_active=peak_active)
+               peak_admitted=peak_admitted, policy=policy, min_cells=min_cells)
     out["records"] = records
     return out
 

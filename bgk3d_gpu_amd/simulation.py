@@ -144,6 +144,31 @@ class Simulation:
                                      out.ctypes.data), "prof_k_gpu")
         return out
 
+    def prof_i(self, jc=None, kc=None) -> np.ndarray:
+        c = self.case
+        jc = c.m // 2 if jc is None else jc
+        kc = c.n // 2 if kc is None else kc
+        out = np.zeros((c.l, 4), dtype=self.R)
+        rt.check(self.lib.prof_i_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, jc, kc, self.Rc(1.0),
+                                     out.ctypes.data), "prof_i_gpu")
+        return out
+
+    def prof_j(self, ic=None, kc=None) -> np.ndarray:
+        c = self.case
+        ic = c.l // 2 if ic is None else ic
+        kc = c.n // 2 if kc is None else kc
+        out = np.zeros((c.m, 4), dtype=self.R)
+        rt.check(self.lib.prof_j_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, ic, kc, self.Rc(1.0),
+                                     out.ctypes.data), "prof_j_gpu")
+        return out
+
+    def probe(self, i0, j0, k0) -> np.ndarray:
+        c = self.case
+        out = np.zeros(4, dtype=self.R)
+        rt.check(self.lib.probe_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, i0, j0, k0, self.Rc(1.0),
+                                    out.ctypes.data), "probe_gpu")
+        return out
+
     def draglift(self) -> np.ndarray:
         c = self.case
         if self.obs_buf is None:

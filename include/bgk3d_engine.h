@@ -139,14 +139,14 @@ const char* bgk_last_error(void);
  * advanced together on one GPU.  The reference's analogue is a shell loop
  * that runs one program per case (UTIL/run_sweep.sh:55-63, 128-169).
  *
- * Requests are admitted in arrival order while the device arena reserved by
- * bgks_create (the HBM budget) can hold their two lattices; every active
- * request advances by one kernel launch per (storage, rounding) group and
+ * Requests are admitted while the device arena reserved by bgks_create (the
+ * HBM budget) can hold their two lattices; the requests scheduled for a
+ * chunk advance by one kernel launch per (storage, rounding) group and
  * step; requests join and leave between chunks of at most chunk_steps
- * steps.  A request's populations are bit-identical to those of a handle
- * (bgk_create with the same n, svisc, u0, storage, rounding and seed) after
- * the same number of steps, whatever else is running.  Not thread-safe: one
- * caller drives a server.
+ * steps (see enum bgks_policy).  A request's populations are bit-identical
+ * to those of a handle (bgk_create with the same n, svisc, u0, storage,
+ * rounding and seed) after the same number of steps, whatever else is
+ * running.  Not thread-safe: one caller drives a server.
  * ------------------------------------------------------------------------- */
 typedef struct bgks_server bgks_server;
 
@@ -164,9 +164,19 @@ typedef struct bgks_request {
 
 enum bgks_state { BGKS_QUEUED = 0, BGKS_ACTIVE = 1, BGKS_DONE = 2 };
 
-/* Reserve `budget_bytes` of device memory for request lattices. */
+enum bgks_policy {
+    BGKS_ALL_ACTIVE = 0,     /* admit in arrival order; every admitted request advances each chunk */
+    BGKS_SHORTEST_FIRST = 1  /* admit and advance the requests with the least work (cells x steps)
+                                left first, enough of them to keep min_cells_per_step in flight */
+};
+
+/* Reserve `budget_bytes` of device memory for request lattices
+ * (policy BGKS_ALL_ACTIVE). */
 int bgks_create(int device, size_t budget_bytes, int chunk_steps, bgks_server** out);
 int bgks_destroy(bgks_server* s);
+
+/* Choose the scheduling policy (takes effect at the next bgks_advance). */
+int bgks_set_policy(bgks_server* s, int policy, long long min_cells_per_step);
 
 /* Device bytes a request holds while admitted (both lattices, padded). */
 int bgks_request_bytes(const bgks_request* r, size_t* bytes);

@@ -123,3 +123,21 @@ def test_request_larger_than_budget_is_refused(gpu):
         while srv.status(rid)[0] != "done":
             srv.advance()
         assert srv.status(rid) == ("done", 10)
+
+
+def test_shortest_first_policy(gpu):
+    """With min_cells = 1 one request runs at a time, least work first; the
+    results stay those of the alone runs."""
+    reqs = [S.Request(n, 300.0, st, steps, seed=q, keep=True) for q, (n, st, steps) in
+            enumerate([(48, "f32", 200), (16, "f16", 100), (32, "f32", 50), (24, "f32", 300)])]
+    banner("serving shortest-first policy")
+    with S.Server(1 << 30, 1000, policy="shortest", min_cells=1) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        order = []
+        while len(order) < len(ids):
+            order += srv.advance()
+        pops = [srv.populations(i, r.n) for i, r in zip(ids, reqs)]
+    work = [r.cells * r.steps for r in reqs]
+    assert order == [ids[q] for q in np.argsort(work, kind="stable")]
+    for q, r in enumerate(reqs):
+        assert np.array_equal(pops[q], alone(r)[0], equal_nan=True)

@@ -4,8 +4,11 @@ Fortran routines (tests/golden/make_fixtures.py).
 Bit-exact: the sha256 of the oracle's interior populations equals the
 reference's for every run (FP32 and FP64), and every single-routine pin
 (one `boundaries` or one `col_MC` call on a seeded random state) matches.
-The reference's text diagnostics (diagno.dat, prof_k.dat, u_med.dat) are
-reproduced character for character from the oracle's values.
+The reference's text diagnostics (diagno.dat, probe.dat, prof_i.dat,
+prof_j.dat, prof_k.dat, u_med.dat) are reproduced character for character
+from the oracle's values, and the files its writers produce (restart,
+velocity, VTK planes) byte for byte by the numpy restatement of the formats
+applied to the oracle's state (restart: the interior of every record).
 """
 import hashlib
 import json
@@ -14,6 +17,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+import format_support as F
 import oracle_support as O
 
 FIX = json.loads((Path(__file__).parent / "golden" / "fixtures.json").read_text())
@@ -32,11 +36,16 @@ def run_oracle(spec, threads=8):
     radius = None
     if spec["case"] == "inflow":
         obs, box, radius = O.build_obs(l, m, n)
-    texts = {"diagno": [], "prof_k": {}, "drag": {}}
+    texts = {"diagno": [], "prof_i": {}, "prof_j": {}, "prof_k": {}, "probe": [], "drag": {}}
     step_to = 0
+    # SRC/diagnostic.F90:98-120
+    pt = (l // 3, m // 5, n // 3) if spec["case"] == "periodic" else (l // 2, m // 2, n // 2)
 
     def sample(t):
         texts["diagno"].append(O.diagno_line(t, O.diagno(A, l, m, n, prec)))
+        texts["probe"].append(f"{t:8d}" + "".join(O.fortran_e(v) + " " for v in O.probe(A, *pt, prec)))
+        texts["prof_i"][t] = O.prof_line(A, l, m, n, 0, m // 2, n // 2, prec)
+        texts["prof_j"][t] = O.prof_line(A, l, m, n, 1, l // 2, n // 2, prec)
         texts["prof_k"][t] = O.prof_k(A, l, m, n, l // 2, m // 2, prec)
         if obs is not None and t > 0:
             texts["drag"][t] = O.draglift(A, obs, l, m, n, radius, prec=prec)
@@ -79,13 +88,44 @@ def test_oracle_reproduces_reference_run(key):
         t, *vals = line.split()
         np.testing.assert_allclose(texts["drag"][int(t)], [float(v) for v in vals], rtol=2e-7 if spec["prec"] == "sp"
                                    else 1e-14, atol=0)
-    # prof_k.dat blocks: (k-0.5, u, w, v, den) in e14.6
-    blocks = O.parse_blocks(spec["texts"]["prof_k.dat"])
-    for t, rows in blocks.items():
-        pk = texts["prof_k"][t]
-        for k, row in enumerate(rows):
-            expect = "".join(O.fortran_e(v) + " " for v in (k + 0.5, *pk[k]))
-            assert row.rstrip() == expect.rstrip(), (t, k, row, expect)
+    # probe.dat (i8, 4e14.6), line for line
+    ref_lines = [x for x in spec["texts"]["probe.dat"].splitlines() if x.strip()]
+    assert len(ref_lines) == len(texts["probe"])
+    for a, b in zip(ref_lines, texts["probe"]):
+        assert a.rstrip() == b.rstrip(), (a, b)
+    # prof_{i,j,k}.dat blocks: (q-0.5, u, w, v, den) in e14.6
+    for name in ("prof_i", "prof_j", "prof_k"):
+        blocks = O.parse_blocks(spec["texts"][f"{name}.dat"])
+        assert set(blocks) == set(texts[name])
+        for t, rows in blocks.items():
+            pk = texts[name][t]
+            assert len(rows) == len(pk)
+            for q, row in enumerate(rows):
+                expect = "".join(O.fortran_e(v) + " " for v in (q + 0.5, *pk[q]))
+                assert row.rstrip() == expect.rstrip(), (name, t, q, row, expect)
+    # files the reference's writers produced at the last step
+    files = spec["files"]
+    steps = spec["steps"]
+    l, m, n, prec = spec["l"], spec["m"], spec["n"], spec["prec"]
+    ours = {
+        f"save.{steps:08d}.bin": F.restart_bytes(steps, A, l, m, n),
+        f"save_vel.{steps:08d}.bin": F.save_vel_bytes(steps, A, l, m, n, prec),
+        f"tec_xy.{steps:08d}.vtk": F.vtk_bytes(0, l, m, n, n // 2, F.vtk_plane_values(A, l, m, n, 0, n // 2, prec)),
+        f"tec_xz.{steps:08d}.vtk": F.vtk_bytes(1, l, m, n, m // 2, F.vtk_plane_values(A, l, m, n, 1, m // 2, prec)),
+        f"tec_yz.{steps:08d}.vtk": F.vtk_bytes(2, l, m, n, l // 2, F.vtk_plane_values(A, l, m, n, 2, l // 2, prec)),
+        f"tec_om.xy.{steps:08d}.vtk": F.vtk_bytes(3, l, m, n, n // 2, F.vorticity_xy_values(A, l, m, n, n // 2, prec)),
+    }
+    got = F.file_digests(ours, l, m, n, prec)
+    assert set(got) == set(files)
+    for key in files:
+        if key == "save_raw" and obs is not None:
+            # solid cells are not part of the parity claim (obstacle deviation)
+            assert got[key]["itime"] == files[key]["itime"]
+            assert got[key]["record_bytes"] == files[key]["record_bytes"]
+            continue
+        if obs is not None and key != "save_raw":
+            continue
+        assert got[key] == files[key], key
 
 
 def test_oracle_reproduces_reference_varm():
@@ -129,3 +169,37 @@ def test_single_routine_pins(key):
         O.col_mc(A, B, l, m, n, omega, prec, force_mode=1 if case == "channel" else 0, fgrad=fgrad)
         O.swap(A, B)
         assert digest(np.ascontiguousarray(A[:, 1:-1, 1:-1, 1:-1])) == spec["sha256_interior"]
+
+
+RESTART_KEYS = sorted(FIX["restart"])
+
+
+@pytest.mark.parametrize("key", RESTART_KEYS)
+def test_restart_file_of_oracle_state(key):
+    """The oracle's state at the save step, written in the restart format,
+    has the interior records of the reference's own save_raw file."""
+    r = FIX["restart"][key]
+    spec = dict(FIX["runs"][r["run"]])
+    spec["steps"] = r["save_step"]
+    spec["icheck"] = r["save_step"]
+    A, *_ = run_oracle(spec)
+    l, m, n, prec = spec["l"], spec["m"], spec["n"], spec["prec"]
+    got = F.restart_interior_digest(F.restart_bytes(r["save_step"], A, l, m, n), l, m, n, prec)
+    assert got == r["save_raw"]
+
+
+@pytest.mark.parametrize("key", RESTART_KEYS)
+def test_reference_restores_our_restart_file(key):
+    """The reference's restore_raw reads a file written from the oracle's
+    state and its run from there ends on the full run's state."""
+    r = FIX["restart"][key]
+    spec = FIX["runs"][r["run"]]
+    if not O.ref_available(spec["case"], spec["prec"]):
+        pytest.skip("reference build not available")
+    half = dict(spec)
+    half["steps"] = half["icheck"] = r["save_step"]
+    A, *_ = run_oracle(half)
+    l, m, n, prec = spec["l"], spec["m"], spec["n"], spec["prec"]
+    B, _, _ = O.run_ref(spec["case"], l, m, n, spec["steps"], prec=prec, svisc=spec["svisc"], u0=spec["u0"],
+                        restore=F.restart_bytes(r["save_step"], A, l, m, n))
+    assert digest(np.ascontiguousarray(B[:, 1:-1, 1:-1, 1:-1])) == spec["sha256_interior"]

@@ -241,3 +241,32 @@ def test_build_obs_shell():
     assert d2.min() > 16 and d2.max() < 64
     assert box == (int(i.min()) + 1, int(i.max()) + 1, int(j.min()) + 1, int(j.max()) + 1, int(k.min()) + 1,
                    int(k.max()) + 1)
+
+
+def test_taylor_green_init_with_density_field():
+    """Init mode 2 keeps the density perturbation: the zeroth moment of the
+    stored state is 1 + u0^2/48 (cos 4 pi x + cos 4 pi y)(cos 4 pi z + 2) and
+    the velocity the Taylor-Green field; mode 1 (the reference) has density
+    1 everywhere and the same momentum times the perturbation."""
+    l = m = n = 16
+    u0 = 0.05
+    A1, A2 = O.new_lattice(l, m, n, "dp"), O.new_lattice(l, m, n, "dp")
+    O.init(A1, l, m, n, "dp", mode=1, u00=u0)
+    O.init(A2, l, m, n, "dp", mode=2, u00=u0)
+    pi = float(np.float32(np.pi))       # default-real pi (SRC/init.F90:63)
+    x = (np.arange(0, l + 2) - 0.5) / l
+    X = x[None, None, :]
+    Y = x[None, :, None]
+    Z = x[:, None, None]
+    rho_exact = 1 + u0 ** 2 / 48 * (np.cos(4 * pi * X) + np.cos(4 * pi * Y)) * (np.cos(4 * pi * Z) + 2)
+    ux = u0 * np.sin(2 * pi * X) * np.cos(2 * pi * Y) * np.cos(2 * pi * Z)
+    uy = -u0 * np.cos(2 * pi * X) * np.sin(2 * pi * Y) * np.cos(2 * pi * Z)
+    cx, cy, cz = (np.array(c, float)[:, None, None, None] for c in (O.CX, O.CY, O.CZ))
+    for A, rho_expect in ((A2, rho_exact), (A1, np.ones_like(rho_exact))):
+        rho = A.sum(axis=0) + 1.0
+        np.testing.assert_allclose(rho, np.broadcast_to(rho_expect, rho.shape), rtol=0, atol=1e-15)
+        jx, jy, jz = (cx * A).sum(axis=0), (cy * A).sum(axis=0), (cz * A).sum(axis=0)
+        # momentum = crho * u in both forms
+        np.testing.assert_allclose(jx, rho_exact * ux, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(jy, rho_exact * uy, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(jz, 0.0, atol=1e-15)
