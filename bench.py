@@ -2,9 +2,10 @@
 """Benchmark of the D3Q19 BGK time step on one MI355X.
 
 Workload (the bandwidth-roofline case): 3-D Taylor-Green vortex, periodic
-256^3, FP32 storage and arithmetic, nu = 0.01 (tau = 0.53), U0 = 0.05.  A
-"step" is one full time step: boundaries (periodic halo) + fused pull-stream
-+ BGK collide, exactly the reference's loop body (SRC/bgk3d.F90:169-196).
+256^3, FP32 storage and arithmetic, nu = 0.01 (tau = 0.53), U0 = 0.05, with
+the matching pressure field in the initial state.  A "step" is one full time
+step: the periodic boundary (folded into the pull) + fused pull-stream + BGK
+collide, the reference's loop body (SRC/bgk3d.F90:169-196).
 
 value  = whole-job MLUPS = l*m*n*steps / time / 1e6 (SRC/finalize.F90:106-107)
 roofline: dominant kernel (the fused collide), algorithmic bytes per launch =
@@ -13,6 +14,11 @@ roofline: dominant kernel (the fused collide), algorithmic bytes per launch =
           launch in the timed region, against the 8 TB/s HBM peak.
 cpu_baseline: the reference's own Fortran CPU routines (oracle/_ref,
           serial build) on a bounded sample of the same workload.
+forms:    FP64 / FP16 / BF16 storage and the reference-compatible ABI.
+serving:  a reduced stream of lid-driven cavity requests (n in 32..192,
+          Re in 100..3200, FP32 or FP16) served by the plain sequential loop
+          and by the batching server: cell updates per second offline, and
+          p50 / p99 latency under Poisson arrivals.
 
 Prints one JSON line.
 """
@@ -121,16 +127,49 @@ def bench_v1(case, prec, steps, warmup):
     return total_ms
 
 
-def bench_engine(case, storage, steps, warmup):
+def bench_engine(case, storage, steps, warmup, want_ke=False, rounding=None):
     from bgk3d_gpu_amd.engine import Engine
 
-    with Engine(case, storage) as eng:
+    if rounding is None:
+        rounding = 1 if storage in ("f16", "bf16") else 0
+    with Engine(case, storage, rounding=rounding) as eng:
+        ke0 = eng.diagno()[4] if want_ke else None
         eng.step(warmup)
         eng.sync()
         total_ms, kern_ms = eng.time_steps(steps)
         diag = eng.diagno()
         info = eng.info()
+    if want_ke:
+        info["ke0"] = ke0
     return total_ms, kern_ms, diag, info
+
+
+def bench_serving(count, steps, budget_gb=32.0, load=0.8):
+    """Offline cell updates per second and online p50/p99 latency of the
+    plain sequential loop and the batching server on a reduced request mix."""
+    from bgk3d_gpu_amd import serving as S
+
+    budget = int(budget_gb * (1 << 30))
+    reqs = S.make_workload(count, steps)
+    S.run_batched(S.make_workload(2, 10, size_seed=7), budget)  # warm-up
+    S.run_sequential(S.make_workload(2, 10, size_seed=7))
+    seq = S.run_sequential(reqs)
+    bat = S.run_batched(reqs, budget, policy="shortest", min_cells=16 << 20)
+    rate = load * count / seq["makespan_s"]
+    on = S.make_workload(count, steps, rate=rate)
+    seq_on = S.run_sequential(on, online=True)
+    bat_on = S.run_batched(on, budget, online=True, policy="shortest", min_cells=16 << 20)
+    r3 = lambda x: round(float(x), 3)  # noqa: E731
+    return {
+        "requests": count, "steps": steps, "cell_steps": int(seq["cell_steps"]),
+        "sequential_mlups": round(seq["throughput_mlups"], 1), "batched_mlups": round(bat["throughput_mlups"], 1),
+        "throughput_ratio": r3(bat["throughput_mlups"] / seq["throughput_mlups"]),
+        "online_rate_per_s": r3(rate),
+        "p50_s": {"sequential": r3(seq_on["latency_p50_s"]), "batched": r3(bat_on["latency_p50_s"])},
+        "p99_s": {"sequential": r3(seq_on["latency_p99_s"]), "batched": r3(bat_on["latency_p99_s"])},
+        "unstable_requests": len(seq["unstable"]),
+        "policy": "shortest remaining work first, >= 16M cells per step",
+    }
 
 
 def main():
@@ -144,6 +183,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP64/FP16/BF16/v1 side measurements")
+    ap.add_argument("--serving-count", type=int, default=16, help="requests in the serving leg (0: skip)")
+    ap.add_argument("--serving-steps", type=int, default=5000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -155,13 +196,17 @@ def main():
     from bgk3d_gpu_amd.cases import Case
 
     n = args.size
-    case = Case("periodic", n, n, n, svisc=0.01, u0=0.05, init=1)
+    case = Case("periodic", n, n, n, svisc=0.01, u0=0.05, init=2)  # Taylor-Green with its pressure field
     storage = "f32" if args.prec == "sp" else "f64"
     workload = f"taylor_green_{n}^3_{'fp32' if args.prec == 'sp' else 'fp64'}"
     log(f"workload {workload} steps={args.steps} warmup={args.warmup}")
-    total_ms, kern_avg_ms, diag, info = bench_engine(case, storage, args.steps, args.warmup)
+    total_ms, kern_avg_ms, diag, info = bench_engine(case, storage, args.steps, args.warmup, want_ke=True)
     if not np.all(np.isfinite(diag)):
         raise SystemExit(f"non-finite diagnostics {diag}")
+    # kinetic energy against the viscous decay exp(-2 nu |k|^2 t), |k|^2 = 3 (2 pi / n)^2
+    t_end = args.warmup + args.steps
+    ke_ratio = diag[4] / info["ke0"]
+    ke_analytic = float(np.exp(-2 * case.svisc * 3 * (2 * np.pi / n) ** 2 * t_end))
     s_bytes = 4 if args.prec == "sp" else 8
     ms_per_step = total_ms / args.steps
     mlups = case.cells * args.steps / (total_ms * 1e-3) / 1e6
@@ -171,14 +216,17 @@ def main():
     forms = {}
     if not args.no_forms:
         fsteps = max(10, args.steps // 4)
-        for st, sb in (("f64", 8), ("f16", 2), ("bf16", 2)):
+        # 16-bit forms store with stochastic rounding (the accurate form);
+        # *_rn is nearest-even rounding
+        for name, st, rnd, sb in (("f64", "f64", 0, 8), ("f16", "f16", 1, 2), ("f16_rn", "f16", 0, 2),
+                                  ("bf16", "bf16", 1, 2), ("bf16_rn", "bf16", 0, 2)):
             if st == storage:
                 continue
-            t, k, d, _ = bench_engine(case, st, fsteps, 5)
-            forms[st] = {"mlups": round(case.cells * fsteps / (t * 1e-3) / 1e6, 1),
-                         "collide_us": round(k * 1e3, 2),
-                         "alg_GBs": round(38 * sb * case.cells / (k * 1e-3) / 1e9, 1)}
-            log(f"{st}: {forms[st]}")
+            t, k, d, _ = bench_engine(case, st, fsteps, 5, rounding=rnd)
+            forms[name] = {"mlups": round(case.cells * fsteps / (t * 1e-3) / 1e6, 1),
+                           "collide_us": round(k * 1e3, 2),
+                           "alg_GBs": round(38 * sb * case.cells / (k * 1e-3) / 1e9, 1)}
+            log(f"{name}: {forms[name]}")
         t = bench_v1(case, args.prec, fsteps, 5)
         forms["v1_abi_" + storage] = {"mlups": round(case.cells * fsteps / (t * 1e-3) / 1e6, 1),
                                       "note": "reference layout (nx=l+2), halo kernels per step"}
@@ -189,6 +237,11 @@ def main():
     if not args.no_stream:
         stream = stream_bandwidth(rt.v1("sp"), rt)
         log(f"measured copy bandwidth {stream:.0f} GB/s")
+    serving = None
+    if args.serving_count > 0:
+        log(f"serving: {args.serving_count} requests x {args.serving_steps} steps ...")
+        serving = bench_serving(args.serving_count, args.serving_steps)
+        log(f"serving: {serving}")
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -218,6 +271,8 @@ def main():
                      "frac_of_measured_copy": round(achieved / stream, 4) if stream else None},
         "cpu_baseline": cpu,
         "forms": forms,
+        "ke_decay": {"steps": t_end, "measured": round(float(ke_ratio), 6), "viscous_analytic": round(ke_analytic, 6)},
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

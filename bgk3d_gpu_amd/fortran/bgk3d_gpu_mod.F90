@@ -131,6 +131,33 @@
           real(c_real)          :: out(*)
         end function prof_k_gpu
 
+        integer(c_int) function prof_i_gpu(a,l,m,n,nx,ny,jcoord,kcoord,  &
+                   cte1,out) bind(C, name="prof_i_gpu")
+          import :: c_ptr, c_int, c_real
+          type(c_ptr)           :: a(19)
+          integer(c_int), value :: l,m,n,nx,ny,jcoord,kcoord
+          real(c_real),   value :: cte1
+          real(c_real)          :: out(*)
+        end function prof_i_gpu
+
+        integer(c_int) function prof_j_gpu(a,l,m,n,nx,ny,icoord,kcoord,  &
+                   cte1,out) bind(C, name="prof_j_gpu")
+          import :: c_ptr, c_int, c_real
+          type(c_ptr)           :: a(19)
+          integer(c_int), value :: l,m,n,nx,ny,icoord,kcoord
+          real(c_real),   value :: cte1
+          real(c_real)          :: out(*)
+        end function prof_j_gpu
+
+        integer(c_int) function probe_gpu(a,l,m,n,nx,ny,i0,j0,k0,cte1,   &
+                   out) bind(C, name="probe_gpu")
+          import :: c_ptr, c_int, c_real
+          type(c_ptr)           :: a(19)
+          integer(c_int), value :: l,m,n,nx,ny,i0,j0,k0
+          real(c_real),   value :: cte1
+          real(c_real)          :: out(4)
+        end function probe_gpu
+
         integer(c_int) function bgk_init_host(h,l,m,n,nx,ny,mode,u00)    &
                    bind(C, name="bgk_init_host")
           import :: c_ptr, c_int, c_real

@@ -260,9 +260,26 @@ def run_batched(requests, budget_bytes: int, chunk_steps: int = 100, online: boo
     return out
 
 
-def run_sequential(requests, online: bool = False, progress=None) -> dict:
-    """The plain loop: one request at a time in arrival order, each with its
-    own engine handle, run to completion before the next starts."""
+def _solve_plain_form(r: Request):
+    """One request through the reference-compatible ABI the reference's way:
+    caller-owned arrays in its layout, host f_eq, per step bcond_driven_gpu
+    then col_mc_gpu and the pointer swap.  That ABI has FP32 and FP64 only,
+    so a 16-bit request runs in FP32."""
+    from .simulation import Simulation
+
+    sim = Simulation(r.case(), "dp" if r.storage == "f64" else "sp", poison=False)
+    try:
+        sim.step(r.steps)
+        return np.asarray(sim.diagno(), dtype=float)
+    finally:
+        sim.close()
+
+
+def run_sequential(requests, online: bool = False, progress=None, path: str = "engine") -> dict:
+    """The plain loop: one request at a time in arrival order, run to
+    completion before the next starts.  path 'engine': each request on its own
+    engine handle (the same kernels and storage as the server); 'v1': through
+    the reference-compatible ABI (`_solve_plain_form`)."""
     order = sorted(range(len(requests)), key=lambda q: requests[q].arrival)
     records = [None] * len(requests)
     t0 = clock()
@@ -271,17 +288,20 @@ def run_sequential(requests, online: bool = False, progress=None) -> dict:
         if online:
             _wait_until(t0 + r.arrival)
         t_start = clock()
-        with E.Engine(r.case(), r.storage, rounding=r.rounding if r.storage in ("f16", "bf16") else 0,
-                      seed=r.seed) as eng:
-            eng.step(r.steps)
-            diag = eng.diagno()
+        if path == "v1":
+            diag = _solve_plain_form(r)
+        else:
+            with E.Engine(r.case(), r.storage, rounding=r.rounding if r.storage in ("f16", "bf16") else 0,
+                          seed=r.seed) as eng:
+                eng.step(r.steps)
+                diag = eng.diagno()
         t_done = clock()
         records[q] = dict(diag=diag, submit=t0 + r.arrival, admit=t_start, done=t_done,
                           unstable=not bool(np.all(np.isfinite(diag))))
         if progress and (c + 1) % 4 == 0:
             progress(f"sequential: {c + 1}/{len(requests)} done")
     t_end = clock()
-    out = _summary(requests, records, t0, t_end, "sequential")
+    out = _summary(requests, records, t0, t_end, f"sequential-{path}")
     out["records"] = records
     return out
 

@@ -7,13 +7,14 @@ default_rng(1).
 
 1. Per size: one request run alone on an engine handle (the plain loop's
    unit of work) against many requests of that size in the batching server.
-2. Offline (all requests at t = 0): the plain sequential loop (one engine
-   handle per request, run to completion, arrival order) and the server with
-   both policies ('all': every admitted request advances each chunk;
-   'shortest': least remaining work first, at least `min_cells` cells per
-   step); throughput in cell updates per second (MLUPS).
-3. Online: Poisson arrivals at `load` times the sequential loop's capacity;
-   p50 / p99 latency (completion - arrival) of the three schedulers.
+2. Offline (all requests at t = 0): sequential plain solves -- through the
+   reference-compatible ABI the reference's way (16-bit requests in FP32,
+   the storage that ABI has), and on one engine handle per request -- and
+   the server with both policies ('all': every admitted request advances
+   each chunk; 'shortest': least remaining work first, at least `min_cells`
+   cells per step); throughput in cell updates per second (MLUPS).
+3. Online: Poisson arrivals at `load` times the plain form's capacity;
+   p50 / p99 latency (completion - arrival) of every scheduler.
 Requests whose solution became non-finite are listed as unstable (the same
 requests under every scheduler).
 
@@ -88,28 +89,33 @@ def main(argv=None):
     if not a.no_per_size:
         out["per_size"] = per_size(budget)
 
-    seq = S.run_sequential(reqs, progress=log)
-    log(f"sequential offline: {seq['throughput_mlups']:.0f} MLUPS, makespan {seq['makespan_s']:.2f} s")
-    offline = {"sequential": strip(seq)}
+    seq_v1 = S.run_sequential(reqs, path="v1")
+    log(f"sequential plain form (v1 ABI) offline: {seq_v1['throughput_mlups']:.0f} MLUPS, "
+        f"makespan {seq_v1['makespan_s']:.2f} s")
+    seq = S.run_sequential(reqs)
+    log(f"sequential engine offline: {seq['throughput_mlups']:.0f} MLUPS, makespan {seq['makespan_s']:.2f} s")
+    offline = {"sequential_v1": strip(seq_v1), "sequential_engine": strip(seq)}
     same = True
     for policy in ("all", "shortest"):
         res = S.run_batched(reqs, budget, a.chunk, policy=policy, min_cells=a.min_cells, progress=None)
         log(f"batched-{policy} offline: {res['throughput_mlups']:.0f} MLUPS, makespan {res['makespan_s']:.2f} s, "
             f"p50 {res['latency_p50_s']:.2f} s")
         offline[f"batched_{policy}"] = strip(res)
-        offline[f"speedup_{policy}"] = res["throughput_mlups"] / seq["throughput_mlups"]
+        offline[f"speedup_{policy}_vs_v1"] = res["throughput_mlups"] / seq_v1["throughput_mlups"]
+        offline[f"speedup_{policy}_vs_engine"] = res["throughput_mlups"] / seq["throughput_mlups"]
         same &= all(np.array_equal(sr["diag"], br["diag"], equal_nan=True)
                     for sr, br in zip(seq["records"], res["records"]))
-    offline["diagnostics_identical"] = bool(same)
+    offline["diagnostics_identical_engine_vs_batched"] = bool(same)
     out["offline"] = offline
     if not a.no_online:
-        rate = a.load * a.count / seq["makespan_s"]
+        rate = a.load * a.count / seq_v1["makespan_s"]
         reqs_on = S.make_workload(a.count, a.steps, rate=rate)
-        log(f"online: Poisson rate {rate:.3f}/s (load {a.load} of the sequential capacity)")
+        log(f"online: Poisson rate {rate:.3f}/s (load {a.load} of the plain form's capacity)")
         online = {"rate_per_s": rate, "load": a.load}
-        seq_on = S.run_sequential(reqs_on, online=True)
-        online["sequential"] = strip(seq_on)
-        log(f"online sequential p50/p99 {seq_on['latency_p50_s']:.2f}/{seq_on['latency_p99_s']:.2f} s")
+        for name, path in (("sequential_v1", "v1"), ("sequential_engine", "engine")):
+            res = S.run_sequential(reqs_on, online=True, path=path)
+            online[name] = strip(res)
+            log(f"online {name} p50/p99 {res['latency_p50_s']:.2f}/{res['latency_p99_s']:.2f} s")
         for policy in ("all", "shortest"):
             res = S.run_batched(reqs_on, budget, a.chunk, online=True, policy=policy, min_cells=a.min_cells)
             online[f"batched_{policy}"] = strip(res)
