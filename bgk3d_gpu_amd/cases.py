@@ -18,6 +18,8 @@ BOUNDARIES = ("ldc", "periodic", "channel", "inflow")
 
 # u_inflow is hard-wired as the default-real literal 0.1 (SRC/bcond.inflow.F90:81)
 U_INFLOW_REF = float(np.float32(0.1))
+# namelist cteS default 0.1, a default-real literal (SRC/input.F90:59)
+CTE_S_REF = float(np.float32(0.1))
 
 
 @dataclass
@@ -34,6 +36,8 @@ class Case:
     obstacle: bool = False         # sphere shell (inflow case)
     u_inflow: float = U_INFLOW_REF
     force_axis: int = 1            # channel body-force axis (1 x, 2 y, 3 z)
+    les: bool = False              # Smagorinsky LES relaxation (the reference's -DLES builds)
+    cte_s: float = CTE_S_REF       # Smagorinsky constant (namelist cteS)
     extra: dict = field(default_factory=dict)
 
     def __post_init__(self):

@@ -48,11 +48,13 @@ struct FusedArgs {
     int l, m, n, nx;
     long nxy;
     C omega, fgrad, fx, fy, fz, lidforce;
+    C cte_s;         // Smagorinsky constant (LES variants)
     CollideConsts<C> cst;
     // obstacle link masks over the box [oi0, oi0+oni) x ... (bit d: x - c_d solid)
     const uint32_t* obits;
     int oi0, oj0, ok0, oni, onj, onk;
     int align;       // 0: layout already aligned; else elements per 128-byte line
+    int lend;        // > l: lanes l < i <= lend store zeros (see fused_rows); 0: off
     uint32_t seed;   // stochastic rounding stream (step-dependent)
 };
 
@@ -196,8 +198,20 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
         shift = (int)(first & (long)(p.align - 1));
     }
     const int i = 1 - shift + seg * 64 + lane;
-    if (i < 1 || i > p.l) return;
+    if (i < 1) return;
     const long s = (long)i + (long)p.nx * j + p.nxy * (long)k;
+    if (i > p.l) {
+        // Whole-line stores: when the row's last 128-byte line is only partly
+        // interior, the lanes past i = l fill it (row padding and the next
+        // row's i = 0 halo cell, which a folded x boundary never reads), so
+        // HBM sees full-line writes instead of partial ones.  Only with a
+        // layout whose rows start on a line (lend set by the engine).
+        if (i <= p.lend) {
+#pragma unroll
+            for (int d = 0; d < 19; ++d) store_nt((d < 18 ? p.b[d] : p.a[18]) + s, Rounder<S, 0>::cvt(C(0), 0u));
+        }
+        return;
+    }
 
     C x[19];
     const bool edge = (BY != AX_HALO && (j == 1 || j == p.m)) || (BZ != AX_HALO && (k == 1 || k == p.n));
@@ -206,13 +220,16 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     else
         pull_all<S, BX, BY, BZ, OBST, false>(p, i, j, k, s, x);
 
+    // FMODE bits 0-1: body force (0 uniform, 1/2/3 fgrad*rho along x/y/z);
+    // bit 2: Smagorinsky LES relaxation
+    constexpr bool LES = (FMODE & 4) != 0;
     C fx, fy, fz;
-    force_terms<C, FMODE>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
+    force_terms<C, (FMODE & 3)>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
     C o[19];
     if constexpr (sizeof(S) == 2)
-        bgk_collide_fast(x, p.omega, fx, fy, fz, p.cst, o);
+        bgk_collide_fast<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
     else
-        bgk_collide_ref(x, p.omega, fx, fy, fz, p.cst, o);
+        bgk_collide_ref<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
 
     // stochastic rounding: one hash per cell and step, then one LCG step per
     // store (deterministic: independent of scheduling)

@@ -62,14 +62,42 @@ struct CollideConsts {
     }
 };
 
+__device__ __forceinline__ float dsqrt(float v) { return __builtin_sqrtf(v); }
+__device__ __forceinline__ double dsqrt(double v) { return __builtin_sqrt(v); }
+
+// Smagorinsky relaxation rate of one cell (SRC/col_MC.F90:381-466, LES
+// builds): the non-equilibrium momentum flux P from n = x - e, then
+// Ts = 1/(2 omega1) + sqrt(18 cteS^2 |P| + (1/omega1)^2)/2 and omega = 1/Ts,
+// in the reference's order (the 2.0 is a default-real literal, exact).
+template <typename R>
+__device__ __forceinline__ R les_omega(const R (&x)[19], const R (&e)[19], R omega1, R cte_s)
+{
+    R nn[18];
+#pragma unroll
+    for (int d = 0; d < 18; ++d) nn[d] = x[d] - e[d];
+    const R pxx = nn[0] + nn[1] + nn[2] + nn[3] + nn[4] + nn[9] + nn[10] + nn[11] + nn[12] + nn[13];
+    const R pyy = nn[0] + nn[2] + nn[6] + nn[7] + nn[8] + nn[9] + nn[11] + nn[15] + nn[16] + nn[17];
+    const R pzz = nn[1] + nn[3] + nn[5] + nn[6] + nn[8] + nn[10] + nn[12] + nn[14] + nn[15] + nn[17];
+    const R pxz = -nn[1] + nn[3] + nn[10] - nn[12];
+    const R pxy = -nn[0] + nn[2] + nn[9] - nn[11];
+    const R pyz = +nn[6] - nn[8] + nn[15] - nn[17];
+    const R two = R(2.0f);
+    const R ptot = dsqrt(pxx * pxx + pyy * pyy + pzz * pzz + (two * pxy * pxy) + (two * pxz * pxz) +
+                         (two * pyz * pyz));
+    const R inv = R(1) / omega1;
+    const R ts = R(1) / (R(2) * omega1) + dsqrt(R(18) * (cte_s * cte_s) * ptot + inv * inv) / R(2);
+    return R(1) / ts;
+}
+
 // Shifted-storage BGK collision of one cell.
 //   x[0..18]  pulled (post-streaming) populations f_i - w_i
-//   omega     relaxation rate 2/(6 nu + 1)
+//   omega     relaxation rate 2/(6 nu + 1) (with LES: the molecular rate
+//             omega1, replaced per cell by les_omega)
 //   fx,fy,fz  the force terms added to the populations with c = +-1
 // out[0..18] = x - omega (x - e) + F, with the reference's term order.
-template <typename R>
+template <typename R, bool LES = false>
 __device__ __forceinline__ void bgk_collide_ref(const R (&x)[19], R omega, R fx, R fy, R fz,
-                                                const CollideConsts<R>& c, R (&o)[19])
+                                                const CollideConsts<R>& c, R (&o)[19], R cte_s = R(0))
 {
     const R uno = R(1), cte1 = c.cte1, cte0 = c.cte0;
     const R rf = c.rf, qf = c.qf, tre = c.tre;
@@ -135,6 +163,8 @@ __device__ __forceinline__ void bgk_collide_ref(const R (&x)[19], R omega, R fx,
     e[17] = rp2 * (-vymz + qymz) + s2;
     e[18] = rp0 * (q0) + s0;
 
+    if constexpr (LES) omega = les_omega(x, e, omega, cte_s);
+
     o[0] = x[0] - omega * (x[0] - e[0]) + fx - fy;
     o[1] = x[1] - omega * (x[1] - e[1]) + fx - fz;
     o[2] = x[2] - omega * (x[2] - e[2]) + fx + fy;
@@ -160,12 +190,12 @@ __device__ __forceinline__ void bgk_collide_ref(const R (&x)[19], R omega, R fx,
 // forms: their results are not bit-comparable with the reference anyway
 // (rounding to 16 bits dominates), and fused multiply-adds cut the VALU work
 // that becomes visible when each cell moves only 76 bytes.
-template <typename R>
+template <typename R, bool LES = false>
 __device__ __forceinline__ void bgk_collide_fast(const R (&x)[19], R omega, R fx, R fy, R fz,
-                                                 const CollideConsts<R>& c, R (&o)[19])
+                                                 const CollideConsts<R>& c, R (&o)[19], R cte_s = R(0))
 {
 #pragma clang fp contract(fast)
-    bgk_collide_ref(x, omega, fx, fy, fz, c, o);
+    bgk_collide_ref<R, LES>(x, omega, fx, fy, fz, c, o, cte_s);
 }
 
 // Force terms of one cell: mode 0 uniform (fx,fy,fz as given), 1/2/3 the

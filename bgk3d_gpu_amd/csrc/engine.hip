@@ -252,33 +252,43 @@ void launch_fused(bgk_handle& h)
                        h.stream, p, rm);
 }
 
-template <typename S, int SR>
-void dispatch_fused(bgk_handle& h)
+// L = 4 selects the Smagorinsky LES relaxation (FMODE bit 2)
+template <typename S, int SR, int L>
+void dispatch_fused_les(bgk_handle& h)
 {
     const bgk_config& c = h.cfg;
     switch (c.boundary) {
-        case BGK_BC_DRIVEN: launch_fused<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>(h); break;
+        case BGK_BC_DRIVEN: launch_fused<S, AX_WALL, AX_WALL, AX_LID, false, L, SR>(h); break;
         case BGK_BC_PERIODIC:
             if (h.has_obs)
-                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_PERIODIC, true, 0, SR>(h);
+                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_PERIODIC, true, L, SR>(h);
             else
-                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_PERIODIC, false, 0, SR>(h);
+                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_PERIODIC, false, L, SR>(h);
             break;
         case BGK_BC_CHANNEL:
             if (c.force_axis == 2)
-                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 2, SR>(h);
+                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 2 | L, SR>(h);
             else if (c.force_axis == 3)
-                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 3, SR>(h);
+                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 3 | L, SR>(h);
             else
-                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 1, SR>(h);
+                launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 1 | L, SR>(h);
             break;
         default:
             if (h.has_obs)
-                launch_fused<S, AX_HALO, AX_PERIODIC, AX_PERIODIC, true, 0, SR>(h);
+                launch_fused<S, AX_HALO, AX_PERIODIC, AX_PERIODIC, true, L, SR>(h);
             else
-                launch_fused<S, AX_HALO, AX_PERIODIC, AX_PERIODIC, false, 0, SR>(h);
+                launch_fused<S, AX_HALO, AX_PERIODIC, AX_PERIODIC, false, L, SR>(h);
             break;
     }
+}
+
+template <typename S, int SR>
+void dispatch_fused(bgk_handle& h)
+{
+    if (h.cfg.les)
+        dispatch_fused_les<S, SR, 4>(h);
+    else
+        dispatch_fused_les<S, SR, 0>(h);
 }
 
 // queue the boundary work and the fused collide of one step, then swap
@@ -345,6 +355,7 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
         return fail(BGK_EINVAL, "obstacles are supported with inflow or periodic boundaries");
     if (!(c.svisc > 0.0)) return fail(BGK_EINVAL, "svisc must be positive");
     if (c.force_axis < 1 || c.force_axis > 3) return fail(BGK_EINVAL, "force_axis %d", c.force_axis);
+    if (c.les != 0 && c.les != 1) return fail(BGK_EINVAL, "les %d", c.les);
     h.cfg = c;
     h.esize = storage_bytes(c.storage);
     h.A = (int)(128 / h.esize);
@@ -371,6 +382,7 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
         h.lidforce = (float)u00 / 6.0f;
     }
     h.u_inflow = c.u_inflow < 0 ? (double)0.1f : K(c.u_inflow);
+    h.cte_s = c.cte_s < 0 ? (double)0.1f : K(c.cte_s);
 
     return 0;
 }
@@ -393,6 +405,8 @@ void bgk_config_default(bgk_config* c, int l, int m, int n)
     c->obs_ic = c->obs_jc = c->obs_kc = c->obs_radius = -1;
     c->u_inflow = -1.0;
     c->force_axis = 1;
+    c->les = 0;
+    c->cte_s = -1.0;
 }
 
 const char* bgk_last_error(void) { return last_error(); }

@@ -26,7 +26,7 @@ struct bgk_handle {
     size_t bytes = 0;
     int cur = 0;
     hipStream_t stream = nullptr;
-    double omega = 0, fgrad = 0, lidforce = 0, u_inflow = 0;
+    double omega = 0, fgrad = 0, lidforce = 0, u_inflow = 0, cte_s = 0;
     // obstacle
     bool has_obs = false;
     int ic = 0, jc = 0, kc = 0, radius = 0;
@@ -143,6 +143,7 @@ FusedArgs<S> fused_args(const bgk_handle& h, int cur)
     p.fgrad = (C)h.fgrad;
     p.fx = p.fy = p.fz = C(0);
     p.lidforce = (C)h.lidforce;
+    p.cte_s = (C)h.cte_s;
     p.cst = CollideConsts<C>::shifted();
     p.obits = h.obits;
     p.oi0 = h.ob[0];
@@ -152,6 +153,9 @@ FusedArgs<S> fused_args(const bgk_handle& h, int cur)
     p.onj = h.ob[4];
     p.onk = h.ob[5];
     p.align = 0;
+    // rows start on a 128-byte line; with the x boundary folded into the
+    // pull the cells after i = l up to that line's end are never read
+    p.lend = h.cfg.boundary == BGK_BC_INFLOW ? 0 : (int)round_up(h.cfg.l, h.A);
     p.seed = step_seed(h.cfg.seed, h.steps);
     return p;
 }

@@ -74,7 +74,8 @@ void launch_face(K kern, long count, const char* name, Args... args)
 // from the output addresses, whatever padding (ipad) the caller chose.
 template <int FM>
 void launch_col(const Lattice19<real_t>& A, const Lattice19<real_t>& B, const Geometry& g, real_t omega,
-                real_t fgrad, real_t ux, real_t uy, real_t uz, const CollideConsts<real_t>& c)
+                real_t fgrad, real_t ux, real_t uy, real_t uz, const CollideConsts<real_t>& c,
+                real_t cte_s = real_t(0))
 {
     if (g.l <= 0 || g.m <= 0 || g.n <= 0) return;
     FusedArgs<real_t> p{};
@@ -91,9 +92,11 @@ void launch_col(const Lattice19<real_t>& A, const Lattice19<real_t>& B, const Ge
     p.fy = uy;
     p.fz = uz;
     p.lidforce = 0;
+    p.cte_s = cte_s;
     p.cst = c;
     p.obits = nullptr;
     p.align = (int)(128 / sizeof(real_t));
+    p.lend = 0;  // the caller's halo and padding are left untouched
     p.seed = 0;
     // when every row shares one alignment the exact shift sizes the grid
     int span = g.l + p.align - 1;
@@ -184,6 +187,21 @@ void col_mc_force_gpu(real_t** a, real_t** b, int l, int m, int n, int nx, int n
         case 2: launch_col<2>(A, B, g, omega, fgrad, 0, 0, 0, c); break;
         case 3: launch_col<3>(A, B, g, omega, fgrad, 0, 0, 0, c); break;
         default: launch_col<0>(A, B, g, omega, 0, 0, 0, 0, c); break;
+    }
+}
+
+void col_mc_les_gpu(real_t** a, real_t** b, int l, int m, int n, int nx, int ny, real_t omega1, real_t cte_s,
+                    real_t fgrad, int axis)
+{
+    Lattice19<real_t> A = lat_from(a), B = lat_from(b);
+    B.p[18] = A.p[18];
+    const Geometry g = geom(l, m, n, nx, ny);
+    const auto c = CollideConsts<real_t>::shifted();
+    switch (axis) {
+        case 1: launch_col<5>(A, B, g, omega1, fgrad, 0, 0, 0, c, cte_s); break;
+        case 2: launch_col<6>(A, B, g, omega1, fgrad, 0, 0, 0, c, cte_s); break;
+        case 3: launch_col<7>(A, B, g, omega1, fgrad, 0, 0, 0, c, cte_s); break;
+        default: launch_col<4>(A, B, g, omega1, 0, 0, 0, 0, c, cte_s); break;
     }
 }
 

@@ -24,7 +24,8 @@ class BGKConfig(ctypes.Structure):
                 ("u0", ctypes.c_double), ("init", ctypes.c_int), ("obstacle", ctypes.c_int),
                 ("obs_ic", ctypes.c_int), ("obs_jc", ctypes.c_int), ("obs_kc", ctypes.c_int),
                 ("obs_radius", ctypes.c_int), ("u_inflow", ctypes.c_double), ("force_axis", ctypes.c_int),
-                ("device", ctypes.c_int), ("seed", ctypes.c_uint)]
+                ("device", ctypes.c_int), ("seed", ctypes.c_uint), ("les", ctypes.c_int),
+                ("cte_s", ctypes.c_double)]
 
 
 def lib() -> ctypes.CDLL:
@@ -63,7 +64,7 @@ def _check(rc, what):
 
 
 def make_config(case: Case, storage: str = "f32", rounding: int = 0, obstacle: int | None = None,
-                centre=None, radius=None, u_inflow=None, device=0, seed=0) -> BGKConfig:
+                centre=None, radius=None, u_inflow=None, device=0, seed=0, les=None, cte_s=None) -> BGKConfig:
     c = BGKConfig()
     lib().bgk_config_default(ctypes.byref(c), case.l, case.m, case.n)
     c.boundary = BOUNDARY[case.boundary]
@@ -81,6 +82,8 @@ def make_config(case: Case, storage: str = "f32", rounding: int = 0, obstacle: i
     c.force_axis = case.force_axis
     c.device = device
     c.seed = seed
+    c.les = 1 if (case.les if les is None else les) else 0
+    c.cte_s = case.cte_s if cte_s is None else cte_s
     return c
 
 

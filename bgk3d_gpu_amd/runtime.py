@@ -45,6 +45,7 @@ def v1(prec: str = "sp") -> ctypes.CDLL:
         lib.col_mc_gpu.argtypes = [vp] * 37 + [ci] * 5 + [R] * 12
         lib.col_mc_gpu.restype = None
         lib.col_mc_force_gpu.argtypes = [vp, vp] + [ci] * 5 + [R, R, ci]
+        lib.col_mc_les_gpu.argtypes = [vp, vp] + [ci] * 5 + [R, R, R, ci]
         lib.col_mc_force_gpu.restype = None
         for f in ("bcond_periodic_gpu", "bcond_channel_gpu"):
             getattr(lib, f).argtypes = [vp] + [ci] * 5

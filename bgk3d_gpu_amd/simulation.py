@@ -99,7 +99,10 @@ class Simulation:
 
     def collision(self):
         c, L, R, W = self.case, self.lib, self.Rc, self.R
-        if c.boundary == "channel":
+        if c.les:
+            L.col_mc_les_gpu(self.pa, self.pb, c.l, c.m, c.n, self.nx, self.ny, R(self.omega), R(c.cte_s),
+                             R(self.fgrad), c.force_axis if c.boundary == "channel" else 0)
+        elif c.boundary == "channel":
             L.col_mc_force_gpu(self.pa, self.pb, c.l, c.m, c.n, self.nx, self.ny, R(self.omega), R(self.fgrad),
                                c.force_axis)
         else:

@@ -54,6 +54,10 @@ typedef struct bgk_config {
     int force_axis;       /* channel body-force axis 1|2|3 (FORCING_Y/Z select 2/3) */
     int device;           /* HIP device ordinal */
     unsigned seed;        /* stochastic-rounding stream */
+    int les;              /* 1: Smagorinsky LES relaxation (the reference's -DLES builds,
+                             SRC/col_MC.F90:381-466) */
+    double cte_s;         /* Smagorinsky constant (namelist cteS); < 0 = reference default
+                             (default-real 0.1, SRC/input.F90:59) */
 } bgk_config;
 
 typedef struct bgk_handle bgk_handle;

@@ -108,6 +108,14 @@ void gpu_device_sync(void);
 void col_mc_force_gpu(bgk_real_t** a, bgk_real_t** b, int l, int m, int n, int nx, int ny, bgk_real_t omega,
                       bgk_real_t fgrad, int axis);
 
+/* col_mc_force_gpu with the Smagorinsky LES relaxation of the reference's
+ * LES builds: per cell omega = 1/Ts, Ts = 1/(2 omega1) + sqrt(18 cteS^2 |P| +
+ * (1/omega1)^2)/2 from the non-equilibrium momentum flux P (cteS: namelist
+ * cteS, default 0.1); axis 0 = no body force.  The reference's native path
+ * has no LES.  Replaces SRC/col_MC.F90:381-466 (with -DLES). */
+void col_mc_les_gpu(bgk_real_t** a, bgk_real_t** b, int l, int m, int n, int nx, int ny, bgk_real_t omega1,
+                    bgk_real_t cte_s, bgk_real_t fgrad, int axis);
+
 /* Global means over interior cells: out = {<u>, <v>, <w>, <rho>, <0.5|u|^2>}
  * (diagno.dat column order), FP64 accumulation; cte1 = 1 for shifted storage.
  * Replaces the host loop of SRC/diagno.F90:86-138 (and its full D2H copy,

@@ -17,8 +17,10 @@
 
 #define SINf32 sinf
 #define COSf32 cosf
+#define SQRTf32 sqrtf
 #define SINf64 sin
 #define COSf64 cos
+#define SQRTf64 sqrt
 
 #define R float
 #define SFX f32
@@ -83,7 +85,7 @@ long oracle_build_obs(int *obs, int l, int m, int n, int *box, int *radius)
 #define RUN_IMPL(R, SFX)                                                                               \
     void oracle_run_##SFX(R **a, R **b, int l, int m, int n, int nx, int ny, int bc, R omega, R u00,    \
                          int force_mode, R fgrad, R u_inflow, const int *obs, const int *box, int steps, \
-                         int nthreads)                                                                 \
+                         R cteS, int nthreads)                                                         \
     {                                                                                                  \
         for (int t = 0; t < steps; ++t) {                                                              \
             if (bc == 0) oracle_bcond_driven_##SFX(a, l, m, n, nx, ny, u00);                           \
@@ -93,7 +95,7 @@ long oracle_build_obs(int *obs, int l, int m, int n, int *box, int *radius)
             if (obs)                                                                                   \
                 oracle_bcond_obs_##SFX(a, obs, l, m, n, nx, ny, box[0], box[1], box[2], box[3], box[4],  \
                                        box[5]);                                                        \
-            oracle_col_mc_##SFX(a, b, l, m, n, nx, ny, omega, force_mode, fgrad, (R)0, (R)0, (R)0,       \
+            oracle_col_mc_##SFX(a, b, l, m, n, nx, ny, omega, force_mode, fgrad, (R)0, (R)0, (R)0, cteS, \
                                 nthreads);                                                             \
             for (int d = 0; d < 18; ++d) {                                                             \
                 R *tmp = a[d];                                                                         \

@@ -147,12 +147,14 @@ void FN(oracle_init_)(R **a, int l, int m, int n, int nx, int ny, int mode, R u0
 /*               reference's non-CHANNEL builds, :147-149);                  */
 /* force_mode 1: CHANNEL forcing forcex = fgrad*rho (:482);                  */
 /* force_mode 2: FORCING_Y forcey = fgrad*rho (:475);                        */
-/* force_mode 3: FORCING_Z forcez = fgrad*rho (:480).                        */
+/* force_mode 3: FORCING_Z forcez = fgrad*rho (:480);                        */
+/* force_mode | 4: LES builds, per-cell Smagorinsky rate from the            */
+/*               non-equilibrium flux with constant cteS (:381-466).         */
 /* k-planes are independent, so the outer loop may run threaded; the result  */
 /* does not depend on the thread count.                                      */
 /* ------------------------------------------------------------------------ */
-void FN(oracle_col_mc_)(R **a, R **b, int l, int m, int n, int nx, int ny, R omega, int force_mode,
-                        R fgrad, R fx_u, R fy_u, R fz_u, int nthreads)
+void FN(oracle_col_mc_)(R **a, R **b, int l, int m, int n, int nx, int ny, R omega1, int force_mode,
+                        R fgrad, R fx_u, R fy_u, R fz_u, R cteS, int nthreads)
 {
     const size_t nxy = (size_t)nx * (size_t)ny;
     const R uno = (R)1, zero = (R)0;
@@ -249,17 +251,41 @@ void FN(oracle_col_mc_)(R **a, R **b, int l, int m, int n, int nx, int ny, R ome
                 const R e17 = rp1 * (-vy + qy) + cte1 * (rp1 - p1);
                 const R e18 = rp2 * (-vymz + qymz) + cte1 * (rp2 - p2);
                 const R e19 = rp0 * (+q0) + cte1 * (rp0 - p0);
+                R omega = omega1;
+                if (force_mode & 4) {
+                    /* LES, :381-466: non-equilibrium parts, momentum flux,
+                     * Smagorinsky relaxation time */
+                    const R n01 = x01 - e01, n02 = x02 - e02, n03 = x03 - e03, n04 = x04 - e04;
+                    const R n05 = x05 - e05, n06 = x06 - e06, n07 = x07 - e07, n08 = x08 - e08;
+                    const R n09 = x09 - e09, n10 = x10 - e10, n11 = x11 - e11, n12 = x12 - e12;
+                    const R n13 = x13 - e13, n14 = x14 - e14, n15 = x15 - e15, n16 = x16 - e16;
+                    const R n17 = x17 - e17, n18 = x18 - e18;
+                    const R Pxx = n01 + n02 + n03 + n04 + n05 + n10 + n11 + n12 + n13 + n14;
+                    const R Pyy = n01 + n03 + n07 + n08 + n09 + n10 + n12 + n16 + n17 + n18;
+                    const R Pzz = n02 + n04 + n06 + n07 + n09 + n11 + n13 + n15 + n16 + n18;
+                    const R Pxz = -n02 + n04 + n11 - n13;
+                    const R Pxy = -n01 + n03 + n10 - n12;
+                    const R Pyz = +n07 - n09 + n16 - n18;
+                    const R Pyx = Pxy, Pzx = Pxz, Pzy = Pyz;
+                    /* 2.0 is a default-real literal (exact in both kinds) */
+                    const R Ptotal = CAT(SQRT, SFX)((Pxx * Pxx) + (Pyy * Pyy) + (Pzz * Pzz) + ((R)2.0f * Pxy * Pyx) +
+                                                    ((R)2.0f * Pxz * Pzx) + ((R)2.0f * Pyz * Pzy));
+                    const R Ts = (R)1 / ((R)2 * omega1) +
+                                 CAT(SQRT, SFX)((R)18 * (cteS * cteS) * Ptotal + ((R)1 / omega1) * ((R)1 / omega1)) /
+                                     (R)2;
+                    omega = (R)1 / Ts;
+                }
                 /* forcing, :472-488 */
                 R forcex = fx_u, forcey = fy_u, forcez = fz_u;
-                if (force_mode == 1) {
+                if ((force_mode & 3) == 1) {
                     forcex = fgrad * rho;
                     forcey = zero;
                     forcez = zero;
-                } else if (force_mode == 2) {
+                } else if ((force_mode & 3) == 2) {
                     forcex = zero;
                     forcey = fgrad * rho;
                     forcez = zero;
-                } else if (force_mode == 3) {
+                } else if ((force_mode & 3) == 3) {
                     forcex = zero;
                     forcey = zero;
                     forcez = fgrad * rho;

@@ -10,7 +10,9 @@
 !
 ! Usage:
 !   ref_harness.x OUTDIR L M N SVISC U0 ITFIN ICHECK INIT_V IPAD JPAD MODE [WRITE]
-! MODE 0: full loop (boundaries -> collision) for ITFIN steps, with the
+! MODE 0: full loop (boundaries -> propagation -> collision, as
+!         SRC/bgk3d.F90:177-179; propagation is empty in the fused build)
+!         for ITFIN steps, with the
 !         reference's diagno/probe/prof_i/prof_j/prof_k (and draglift
 !         under DRAG) every ICHECK, as SRC/diagnostic.F90:98-120 calls them;
 ! MODE 1: one call of `boundaries` only;
@@ -106,6 +108,7 @@
          call system_clock(c0, crate, cmax)
          do itime = itstart+1, itfin
             call boundaries
+            call propagation
             call collision(itime)
             if (mode == 0 .and. mod(itime, icheck) == 0) then
                call diagno(itime)

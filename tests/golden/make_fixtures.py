@@ -15,6 +15,8 @@ by oracle/Makefile, linked with our small harness) and records, per run:
     planes vtk_xy/xz/yz_bin and vtk_om_xy_bin (whole files);
   * a restart pin: the reference's own save_raw at half the steps, restored
     by its restore_raw and run on, ends on the full run's state;
+  * LES runs from the reference's -DLES builds, and the ORIGINAL two-pass
+    build's final state next to the fused build's;
   * for the single-routine pins: a seeded random state pushed through one
     `boundaries` call or one `col_MC` call.
 
@@ -42,6 +44,14 @@ RUNS = [
     ("tgv_32", "periodic", 32, 32, 32, 200, 0.01, 0.05, 1, 100),
     ("channel_64x32x32", "channel", 64, 32, 32, 500, 0.1, 0.02048, 0, 250),
     ("inflow_sphere_100x48x48", "inflow", 100, 48, 48, 300, 0.02, 0.1, 0, 100),
+]
+
+# LES builds (-DLES, SRC/col_MC.F90:381-466; cteS = 0.1): low viscosities
+# where the Smagorinsky rate departs from omega
+RUNS_LES = [
+    ("ldc_20x18x16_les", "ldc", 20, 18, 16, 200, 0.002, 0.1, 0, 100),
+    ("tgv_24x20x16_les", "periodic", 24, 20, 16, 100, 0.001, 0.05, 1, 50),
+    ("channel_24x16x20_les", "channel", 24, 16, 20, 200, 0.01, 0.02, 0, 100),
 ]
 
 # single-routine pins on seeded random states: (name, case, l, m, n, mode)
@@ -74,7 +84,7 @@ def random_state(l, m, n, prec, seed):
 
 
 def main():
-    out = {"generator": "tests/golden/make_fixtures.py", "runs": {}, "unit": {}, "restart": {}}
+    out = {"generator": "tests/golden/make_fixtures.py", "runs": {}, "unit": {}, "restart": {}, "original": {}}
     for name, case, l, m, n, steps, svisc, u0, init_v, icheck in RUNS:
         for prec in ("sp", "dp"):
             if not O.ref_available(case, prec):
@@ -91,6 +101,30 @@ def main():
                 "files": F.file_digests(texts["files"], l, m, n, prec),
             }
             print(name, prec, "done", flush=True)
+    # LES runs
+    for name, case, l, m, n, steps, svisc, u0, init_v, icheck in RUNS_LES:
+        for prec in ("sp", "dp"):
+            if not O.ref_available(case, prec, variant="_les"):
+                raise SystemExit(f"oracle/_ref/{case}_{prec}_les not built")
+            A, texts, _ = O.run_ref(case, l, m, n, steps, prec=prec, svisc=svisc, u0=u0, icheck=icheck,
+                                    init_v=init_v, variant="_les")
+            I = interior_bytes(A)
+            out["runs"][f"{name}_{prec}"] = {
+                "case": case, "l": l, "m": m, "n": n, "steps": steps, "svisc": svisc, "u0": u0, "init": init_v,
+                "icheck": icheck, "prec": prec, "les": True, "sha256_interior": digest(I),
+                "pop_sums": [float(x) for x in I.reshape(19, -1).astype(np.float64).sum(axis=1)],
+                "texts": {k: v for k, v in texts.items() if k not in ("obs", "files")},
+            }
+            print(name, prec, "done", flush=True)
+    # the ORIGINAL two-pass build (movef + col) ends on the fused build's state
+    for name, case, l, m, n, steps, svisc, u0, init_v, icheck in RUNS[:1] + RUNS[2:4]:
+        for prec in ("sp", "dp"):
+            A, _, _ = O.run_ref(case, l, m, n, steps, prec=prec, svisc=svisc, u0=u0, icheck=icheck, init_v=init_v,
+                                variant="_orig")
+            out["original"][f"{name}_{prec}"] = {
+                "sha256_interior": digest(interior_bytes(A)),
+                "equals_fused": digest(interior_bytes(A)) == out["runs"][f"{name}_{prec}"]["sha256_interior"]}
+            print("original", name, prec, out["original"][f"{name}_{prec}"]["equals_fused"], flush=True)
     # restart pin: save_raw at half the steps, restore_raw, run on
     for name, case, l, m, n, steps, svisc, u0, init_v, icheck in RUNS[:1] + RUNS[2:3]:
         for prec in ("sp", "dp"):

@@ -104,14 +104,20 @@ def build_obs(l, m, n):
     return obs, tuple(box), rad.value
 
 
-def col_mc(A, B, l, m, n, omega, prec="sp", force_mode=0, fgrad=0.0, fxyz=(0.0, 0.0, 0.0), nthreads=1):
+# the reference's default Smagorinsky constant: cteS = 0.1, a default-real
+# literal (SRC/input.F90:59), so the DP build sees the float-rounded value
+CTE_S_REF = float(np.float32(0.1))
+
+
+def col_mc(A, B, l, m, n, omega, prec="sp", force_mode=0, fgrad=0.0, fxyz=(0.0, 0.0, 0.0), nthreads=1,
+           les=False, cte_s=CTE_S_REF):
     nx, ny = A.shape[3], A.shape[2]
     ct = ctype(prec)
     f = fn("col_mc", prec)
-    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5 + [ct, ctypes.c_int, ct, ct, ct, ct,
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5 + [ct, ctypes.c_int, ct, ct, ct, ct, ct,
                                                                            ctypes.c_int]
-    f(ptrs(A), ptrs(B), l, m, n, nx, ny, ct(omega), force_mode, ct(fgrad), ct(fxyz[0]), ct(fxyz[1]),
-      ct(fxyz[2]), nthreads)
+    f(ptrs(A), ptrs(B), l, m, n, nx, ny, ct(omega), force_mode | (4 if les else 0), ct(fgrad), ct(fxyz[0]),
+      ct(fxyz[1]), ct(fxyz[2]), ct(cte_s), nthreads)
 
 
 def swap(A, B):
@@ -143,7 +149,7 @@ def bcond(case, A, l, m, n, prec="sp", u00=0.0, u_inflow=U_INFLOW_REF, obs=None,
 
 
 def run(case, A, B, l, m, n, steps, prec="sp", svisc=0.064, u0=0.1, obs=None, box=None, u_inflow=U_INFLOW_REF,
-        nthreads=1):
+        nthreads=1, les=False, cte_s=CTE_S_REF):
     """Reference time loop (SRC/bgk3d.F90:169-196) on host arrays A, B.
 
     Pointer swaps are done on the pointer tables; on return the current
@@ -160,11 +166,11 @@ def run(case, A, B, l, m, n, steps, prec="sp", svisc=0.064, u0=0.1, obs=None, bo
     f = fn("run", prec)
     f.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6 + [ct, ct, ctypes.c_int, ct, ct,
                                                                            ctypes.c_void_p, ctypes.c_void_p,
-                                                                           ctypes.c_int, ctypes.c_int]
+                                                                           ctypes.c_int, ct, ctypes.c_int]
     obs_p = obs.ctypes.data_as(ctypes.c_void_p) if obs is not None else None
     box_p = (ctypes.c_int * 6)(*box) if box is not None else None
-    f(PA, PB, l, m, n, nx, ny, BC_CODE[case], ct(omega), ct(u00), force_mode, ct(fgrad), ct(u_inflow), obs_p,
-      box_p, steps, nthreads)
+    f(PA, PB, l, m, n, nx, ny, BC_CODE[case], ct(omega), ct(u00), force_mode | (4 if les else 0), ct(fgrad),
+      ct(u_inflow), obs_p, box_p, steps, ct(cte_s), nthreads)
     # gather the current lattice back into A
     cur = [None] * 19
     addr = {A[d].ctypes.data: ("A", d) for d in range(19)}
@@ -231,19 +237,20 @@ def draglift(A, obs, l, m, n, radius, u_inflow=U_INFLOW_REF, prec="sp"):
 # compiled reference harness (only where oracle/_ref was built)
 # ---------------------------------------------------------------------------
 
-def ref_available(case="ldc", prec="sp", omp=False) -> bool:
-    return (REF_DIR / f"{case}_{prec}{'_omp' if omp else ''}" / "ref_harness.x").exists()
+def ref_available(case="ldc", prec="sp", omp=False, variant="") -> bool:
+    """variant: '' (fused build), '_orig' (ORIGINAL two-pass build), '_les'."""
+    return (REF_DIR / f"{case}_{prec}{'_omp' if omp else variant}" / "ref_harness.x").exists()
 
 
 def run_ref(case, l, m, n, steps, prec="sp", svisc=0.064, u0=0.1, icheck=None, init_v=0, ipad=0, jpad=0,
-            mode=0, A_in=None, omp=False, threads=None, keep_dir=None, write=False, restore=None):
+            mode=0, A_in=None, omp=False, threads=None, keep_dir=None, write=False, restore=None, variant=""):
     """Run the reference harness; returns (A_out, outdir_texts, stdout).
 
     write=True adds texts["files"] = {name: bytes} of what the reference's
     save_raw, save_vel and VTK writers produced at the last step;
     restore=<bytes> starts from that restart file (the reference's
     restore_raw) instead of init."""
-    exe = REF_DIR / f"{case}_{prec}{'_omp' if omp else ''}" / "ref_harness.x"
+    exe = REF_DIR / f"{case}_{prec}{'_omp' if omp else variant}" / "ref_harness.x"
     d = Path(keep_dir) if keep_dir else Path(tempfile.mkdtemp(prefix="bgkref_"))
     d.mkdir(parents=True, exist_ok=True)
     try:

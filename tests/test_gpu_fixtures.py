@@ -4,7 +4,7 @@
   tests/golden/make_fixtures.py) is reproduced bit for bit -- sha256 of the
   interior populations -- by both the reference-compatible ABI and the
   handle API, in FP32 and FP64.  This includes the lid-driven cavity 64^3,
-  Re = 100, 2000 steps.
+  Re = 100, 2000 steps, and the Smagorinsky LES builds.
 * The reference's only stored result, the centreline profile of a 256^3
   Re = 1000 cavity (TEST/DrivenCavity/OFFLOAD_256/prof_k.dat, written by an
   OpenMP-offload GPU build), is matched within 2e-5 absolute (the lid speed
@@ -37,7 +37,7 @@ def banner(msg):
 
 def case_of(spec):
     return Case(spec["case"], spec["l"], spec["m"], spec["n"], svisc=spec["svisc"], u0=spec["u0"],
-                init=spec["init"], obstacle=spec["case"] == "inflow")
+                init=spec["init"], obstacle=spec["case"] == "inflow", les=spec.get("les", False))
 
 
 def interior_digest(P, spec, dtype):
