@@ -77,11 +77,22 @@ __device__ __forceinline__ void axis_fix(int cd, int s, int lo_hi_n, int& shift,
 
 // EDGE = the row (j,k) touches a y or z boundary; otherwise only the x
 // direction can leave the interior and the j/k checks are compiled out.
+//
+// Addressing: j and k are the same for a whole wave, so the row's byte offset
+// and everything the y and z boundaries decide (wrap, wall) are scalar; each
+// lane adds only its own byte offset i*sizeof(S) (plus an x wrap), so every
+// load and store is a scalar base + one 32-bit lane offset (saddr form) and
+// no 64-bit address arithmetic runs per lane.  A lane whose x neighbour is a
+// wall or a solid cell loads its own opposite population in a second, masked
+// load; its first load stays inside the array (the halo or the solid cell).
 template <typename S, int BX, int BY, int BZ, bool OBST, bool EDGE>
-__device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, int k, long s,
+__device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, int k, long row,
                                          typename Storage<S>::compute (&x)[19])
 {
     typedef typename Storage<S>::compute C;
+    constexpr long ES = (long)sizeof(S);
+    const long rowb = row * ES;                        // scalar: byte offset of cell (0, j, k)
+    const uint32_t vi = (uint32_t)i * (uint32_t)ES;    // per lane
     uint32_t ob = 0;
     if (OBST) {
         const unsigned ri = (unsigned)(i - p.oi0), rj = (unsigned)(j - p.oj0), rk = (unsigned)(k - p.ok0);
@@ -91,26 +102,34 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
 #pragma unroll
     for (int d = 0; d < 19; ++d) {
         const int cx = cx_of(d), cy = cy_of(d), cz = cz_of(d);
-        long src = s - ((long)cx + (long)p.nx * cy + p.nxy * cz);
-        bool bounce = false, top = false;
+        long ds = -((long)p.nx * cy + p.nxy * cz);  // scalar element shift (y, z)
+        int dx = -cx;                                // per-lane element shift (x)
+        bool bounce = false, wall = false, top = false;
         if (cx != 0) {
             int sh = 0;
             axis_fix<BX>(cx, i - cx, p.l, sh, bounce, top);
-            if (BX == AX_PERIODIC) src += (long)sh * p.l;
+            if (BX == AX_PERIODIC) dx += sh * p.l;
         }
         if (EDGE && cy != 0) {
             int sh = 0;
-            axis_fix<BY>(cy, j - cy, p.m, sh, bounce, top);
-            if (BY == AX_PERIODIC) src += (long)sh * p.m * p.nx;
+            axis_fix<BY>(cy, j - cy, p.m, sh, wall, top);
+            if (BY == AX_PERIODIC) ds += (long)sh * p.m * p.nx;
         }
         if (EDGE && cz != 0) {
             int sh = 0;
-            axis_fix<BZ>(cz, k - cz, p.n, sh, bounce, top);
-            if (BZ == AX_PERIODIC) src += (long)sh * p.n * p.nxy;
+            axis_fix<BZ>(cz, k - cz, p.n, sh, wall, top);
+            if (BZ == AX_PERIODIC) ds += (long)sh * p.n * p.nxy;
         }
         if (OBST && d < 18) bounce = bounce || ((ob >> d) & 1u);
-        const S* ptr = bounce ? (p.a[opp_of(d)] + s) : (p.a[d] + src);
-        C v = Storage<S>::load(ptr);
+        const char* own = reinterpret_cast<const char*>(p.a[opp_of(d)]) + rowb;
+        C v;
+        if (wall) {  // y or z wall: the whole wave bounces
+            v = Storage<S>::load(reinterpret_cast<const S*>(own + (uint64_t)vi));
+        } else {
+            const char* base = reinterpret_cast<const char*>(p.a[d]) + rowb + ds * ES;
+            v = Storage<S>::load(reinterpret_cast<const S*>(base + (uint64_t)(vi + (uint32_t)dx * (uint32_t)ES)));
+            if (bounce) v = Storage<S>::load(reinterpret_cast<const S*>(own + (uint64_t)vi));
+        }
         if (BZ == AX_LID && EDGE) {
             if (d == 1 && top) v = v + p.lidforce;
             if (d == 10 && top) v = v - p.lidforce;
@@ -179,7 +198,8 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
                                             uint32_t seed)
 {
     typedef typename Storage<S>::compute C;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr long ES = (long)sizeof(S);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     if (wave >= rm.wpb) return;
     int seg, j;
     if (rm.rpb == 1) {
@@ -199,7 +219,8 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     }
     const int i = 1 - shift + seg * 64 + lane;
     if (i < 1) return;
-    const long s = (long)i + (long)p.nx * j + p.nxy * (long)k;
+    const long row = (long)p.nx * j + p.nxy * (long)k;  // scalar
+    const uint32_t vi = (uint32_t)i * (uint32_t)ES;
     if (i > p.l) {
         // Whole-line stores: when the row's last 128-byte line is only partly
         // interior, the lanes past i = l fill it (row padding and the next
@@ -208,7 +229,10 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
         // layout whose rows start on a line (lend set by the engine).
         if (i <= p.lend) {
 #pragma unroll
-            for (int d = 0; d < 19; ++d) store_nt((d < 18 ? p.b[d] : p.a[18]) + s, Rounder<S, 0>::cvt(C(0), 0u));
+            for (int d = 0; d < 19; ++d)
+                store_nt(reinterpret_cast<S*>(reinterpret_cast<char*>(d < 18 ? p.b[d] : p.a[18]) + row * ES +
+                                              (uint64_t)vi),
+                         Rounder<S, 0>::cvt(C(0), 0u));
         }
         return;
     }
@@ -216,9 +240,9 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     C x[19];
     const bool edge = (BY != AX_HALO && (j == 1 || j == p.m)) || (BZ != AX_HALO && (k == 1 || k == p.n));
     if (edge)
-        pull_all<S, BX, BY, BZ, OBST, true>(p, i, j, k, s, x);
+        pull_all<S, BX, BY, BZ, OBST, true>(p, i, j, k, row, x);
     else
-        pull_all<S, BX, BY, BZ, OBST, false>(p, i, j, k, s, x);
+        pull_all<S, BX, BY, BZ, OBST, false>(p, i, j, k, row, x);
 
     // FMODE bits 0-1: body force (0 uniform, 1/2/3 fgrad*rho along x/y/z);
     // bit 2: Smagorinsky LES relaxation
@@ -227,17 +251,17 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     force_terms<C, (FMODE & 3)>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
     C o[19];
     if constexpr (sizeof(S) == 2)
-        bgk_collide_fast<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
+        bgk_collide_lean<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
     else
         bgk_collide_ref<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
 
     // stochastic rounding: one hash per cell and step, then one LCG step per
     // store (deterministic: independent of scheduling)
     uint32_t r = 0;
-    if (SR) r = hash32(seed ^ ((uint32_t)s * 0x9e3779b1u));
+    if (SR) r = hash32(seed ^ ((uint32_t)(row + i) * 0x9e3779b1u));
 #pragma unroll
     for (int d = 0; d < 19; ++d) {
-        S* dst = (d < 18 ? p.b[d] : p.a[18]) + s;
+        S* dst = reinterpret_cast<S*>(reinterpret_cast<char*>(d < 18 ? p.b[d] : p.a[18]) + row * ES + (uint64_t)vi);
         if (SR) r = lcg_next(r);
         store_nt(dst, Rounder<S, SR>::cvt(o[d], r));
     }

@@ -68,8 +68,8 @@ __host__ __device__ inline uint32_t hash32(uint32_t x)
 }
 
 // Round a float to the storage type.  SR = 0: nearest-even; SR = 1:
-// stochastic with the 32-bit random word r (only its high bits are used,
-// which is where the per-store LCG stream below is strongest).
+// stochastic with the 32-bit random word r, one LCG step per store (its
+// high bits are the strong ones).
 __device__ __forceinline__ uint32_t lcg_next(uint32_t r) { return r * 747796405u + 2891336453u; }
 template <typename S, int SR>
 struct Rounder;
@@ -86,42 +86,36 @@ template <>
 struct Rounder<f16_t, 0> {
     __device__ __forceinline__ static f16_t cvt(float v, uint32_t) { return (f16_t)v; }
 };
+// Stochastic rounding uses gfx950's conversions with a random operand
+// (v_cvt_sr_f16_f32, v_cvt_sr_bf16_f32): one instruction per store.  The
+// word handed over is r ^ (r >> 16), so both halves carry the well-mixed
+// high bits of the LCG stream whichever bits the conversion consumes.
+typedef _Float16 sr_half2 __attribute__((ext_vector_type(2)));
+typedef __bf16 sr_bf162 __attribute__((ext_vector_type(2)));
+
 template <>
 struct Rounder<f16_t, 1> {
-    // Unbiased stochastic rounding to FP16 without branches: add a uniform
-    // fraction of the FP16 spacing at |v| (2^(e-10) in the normal range,
-    // 2^-24 below it; 13 random bits, so the sum is exact in FP32 for the
-    // normal range) away from zero, then convert with round-toward-zero
-    // (v_cvt_pkrtz_f16_f32), which truncates to the spacing in the normal
-    // and the subnormal range alike.  |v| >= 2^15 converts to nearest.
     __device__ __forceinline__ static f16_t cvt(float v, uint32_t r)
     {
-        const uint32_t e = __float_as_uint(v) & 0x7f800000u;
-        const float ulp = __uint_as_float((e > 0x38800000u ? e : 0x38800000u) - 0x05000000u);
-        const float du = (float)(r >> 19) * 1.220703125e-4f * ulp;
-        const float t = v + __builtin_copysignf(du, v);
-        const f16_t sr = __builtin_amdgcn_cvt_pkrtz(t, t)[0];
-        return e >= 0x47000000u ? (f16_t)v : sr;
+        const sr_half2 z = {0, 0};
+        return __builtin_amdgcn_cvt_sr_f16_f32(z, v, r ^ (r >> 16), false)[0];
     }
 };
 template <>
 struct Rounder<bf16_t, 0> {
+    // nearest-even through the hardware conversion (v_cvt_pk_bf16_f32)
     __device__ __forceinline__ static bf16_t cvt(float v, uint32_t)
     {
-        uint32_t u = __float_as_uint(v);
-        if ((u & 0x7f800000U) == 0x7f800000U) return bf16_t{(uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0))};
-        u += 0x7fffU + ((u >> 16) & 1U);
-        return bf16_t{(uint16_t)(u >> 16)};
+        return bf16_t{__builtin_bit_cast(uint16_t, (__bf16)v)};
     }
 };
 template <>
 struct Rounder<bf16_t, 1> {
     __device__ __forceinline__ static bf16_t cvt(float v, uint32_t r)
     {
-        uint32_t u = __float_as_uint(v);
-        if ((u & 0x7f800000U) == 0x7f800000U) return bf16_t{(uint16_t)(u >> 16)};
-        u += (r >> 16);  // add a uniform fraction of one bf16 ulp (top bits of r), then truncate
-        return bf16_t{(uint16_t)(u >> 16)};
+        const sr_bf162 z = {0, 0};
+        const __bf16 h = __builtin_amdgcn_cvt_sr_bf16_f32(z, v, r ^ (r >> 16), false)[0];
+        return bf16_t{__builtin_bit_cast(uint16_t, h)};
     }
 };
 
