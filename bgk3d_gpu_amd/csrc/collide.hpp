@@ -78,13 +78,15 @@ __device__ __forceinline__ void axis_fix(int cd, int s, int lo_hi_n, int& shift,
 // EDGE = the row (j,k) touches a y or z boundary; otherwise only the x
 // direction can leave the interior and the j/k checks are compiled out.
 //
-// Addressing: j and k are the same for a whole wave, so the row's byte offset
-// and everything the y and z boundaries decide (wrap, wall) are scalar; each
-// lane adds only its own byte offset i*sizeof(S) (plus an x wrap), so every
-// load and store is a scalar base + one 32-bit lane offset (saddr form) and
-// no 64-bit address arithmetic runs per lane.  A lane whose x neighbour is a
-// wall or a solid cell loads its own opposite population in a second, masked
-// load; its first load stays inside the array (the halo or the solid cell).
+// Addressing: j and k are the same for a whole wave, so the row's offset and
+// everything the y and z boundaries decide (wrap, wall) are scalar.  For the
+// 16-bit forms, whose kernels are bound by instruction issue, each lane then
+// adds only its own byte offset i*sizeof(S) (plus an x wrap): every load and
+// store is a scalar base + one 32-bit lane offset (saddr form) and no 64-bit
+// address arithmetic runs per lane, except for the directions where a lane
+// may bounce off an x wall or a solid cell, where each lane picks its own
+// address.  The 32/64-bit forms are bound by HBM and keep per-lane 64-bit
+// addresses, which leaves the scalar registers free for higher occupancy.
 template <typename S, int BX, int BY, int BZ, bool OBST, bool EDGE>
 __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, int k, long row,
                                          typename Storage<S>::compute (&x)[19])
@@ -121,14 +123,34 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
             if (BZ == AX_PERIODIC) ds += (long)sh * p.n * p.nxy;
         }
         if (OBST && d < 18) bounce = bounce || ((ob >> d) & 1u);
-        const char* own = reinterpret_cast<const char*>(p.a[opp_of(d)]) + rowb;
         C v;
+        if constexpr (sizeof(S) >= 4) {
+            // 32/64-bit storage is bound by HBM, not by issue: per-lane 64-bit
+            // addresses keep the scalar registers free (higher occupancy)
+            const long s = row + i;
+            const long src = s + ds + dx;
+            const bool b = wall || bounce;
+            v = Storage<S>::load(b ? (p.a[opp_of(d)] + s) : (p.a[d] + src));
+            if (BZ == AX_LID && EDGE) {
+                if (d == 1 && top) v = v + p.lidforce;
+                if (d == 10 && top) v = v - p.lidforce;
+            }
+            x[d] = v;
+            continue;
+        }
+        const char* own = reinterpret_cast<const char*>(p.a[opp_of(d)]) + rowb;
         if (wall) {  // y or z wall: the whole wave bounces
             v = Storage<S>::load(reinterpret_cast<const S*>(own + (uint64_t)vi));
         } else {
             const char* base = reinterpret_cast<const char*>(p.a[d]) + rowb + ds * ES;
-            v = Storage<S>::load(reinterpret_cast<const S*>(base + (uint64_t)(vi + (uint32_t)dx * (uint32_t)ES)));
-            if (bounce) v = Storage<S>::load(reinterpret_cast<const S*>(own + (uint64_t)vi));
+            const S* nptr = reinterpret_cast<const S*>(base + (uint64_t)(vi + (uint32_t)dx * (uint32_t)ES));
+            if ((BX == AX_WALL && cx != 0) || (OBST && d < 18)) {
+                // a lane may bounce: pick its address, one load
+                const S* optr = reinterpret_cast<const S*>(own + (uint64_t)vi);
+                v = Storage<S>::load(bounce ? optr : nptr);
+            } else {
+                v = Storage<S>::load(nptr);
+            }
         }
         if (BZ == AX_LID && EDGE) {
             if (d == 1 && top) v = v + p.lidforce;
@@ -199,7 +221,10 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
 {
     typedef typename Storage<S>::compute C;
     constexpr long ES = (long)sizeof(S);
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    // the 16-bit forms declare the wave index scalar, so that the row offset
+    // and the boundary decisions stay in scalar registers
+    const int wave = sizeof(S) == 2 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (wave >= rm.wpb) return;
     int seg, j;
     if (rm.rpb == 1) {
@@ -229,10 +254,15 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
         // layout whose rows start on a line (lend set by the engine).
         if (i <= p.lend) {
 #pragma unroll
-            for (int d = 0; d < 19; ++d)
-                store_nt(reinterpret_cast<S*>(reinterpret_cast<char*>(d < 18 ? p.b[d] : p.a[18]) + row * ES +
-                                              (uint64_t)vi),
-                         Rounder<S, 0>::cvt(C(0), 0u));
+            for (int d = 0; d < 19; ++d) {
+                S* dst;
+                if constexpr (sizeof(S) >= 4)
+                    dst = (d < 18 ? p.b[d] : p.a[18]) + (row + i);
+                else
+                    dst = reinterpret_cast<S*>(reinterpret_cast<char*>(d < 18 ? p.b[d] : p.a[18]) + row * ES +
+                                               (uint64_t)vi);
+                store_nt(dst, Rounder<S, 0>::cvt(C(0), 0u));
+            }
         }
         return;
     }
@@ -261,7 +291,11 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     if (SR) r = hash32(seed ^ ((uint32_t)(row + i) * 0x9e3779b1u));
 #pragma unroll
     for (int d = 0; d < 19; ++d) {
-        S* dst = reinterpret_cast<S*>(reinterpret_cast<char*>(d < 18 ? p.b[d] : p.a[18]) + row * ES + (uint64_t)vi);
+        S* dst;
+        if constexpr (sizeof(S) >= 4)
+            dst = (d < 18 ? p.b[d] : p.a[18]) + (row + i);
+        else
+            dst = reinterpret_cast<S*>(reinterpret_cast<char*>(d < 18 ? p.b[d] : p.a[18]) + row * ES + (uint64_t)vi);
         if (SR) r = lcg_next(r);
         store_nt(dst, Rounder<S, SR>::cvt(o[d], r));
     }

@@ -93,6 +93,14 @@
           real(c_real),   value :: omega,fgrad
         end subroutine col_mc_force_gpu
 
+        subroutine col_mc_les_gpu(a,b,l,m,n,nx,ny,omega1,cte_s,fgrad,   &
+                   axis) bind(C, name="col_mc_les_gpu")
+          import :: c_ptr, c_int, c_real
+          type(c_ptr)           :: a(19), b(19)
+          integer(c_int), value :: l,m,n,nx,ny,axis
+          real(c_real),   value :: omega1,cte_s,fgrad
+        end subroutine col_mc_les_gpu
+
         integer(c_int) function diagno_gpu(a,l,m,n,nx,ny,cte1,out)       &
                    bind(C, name="diagno_gpu")
           import :: c_ptr, c_int, c_real, c_double
