@@ -144,31 +144,39 @@ def bench_engine(case, storage, steps, warmup, want_ke=False, rounding=None):
     return total_ms, kern_ms, diag, info
 
 
-def bench_serving(count, steps, budget_gb=32.0, load=0.8):
+def bench_serving(count, steps, budget_gb=32.0, load=0.8, min_cells=4 << 20):
     """Offline cell updates per second and online p50/p99 latency of the
-    plain sequential loop and the batching server on a reduced request mix."""
+    sequential plain loop -- the reference-compatible ABI driven the
+    reference's way, one request after the other -- of a sequential loop of
+    engine handles, and of the batching server (shortest remaining work
+    first, at least `min_cells` cells per step) on a reduced request mix."""
     from bgk3d_gpu_amd import serving as S
 
     budget = int(budget_gb * (1 << 30))
     reqs = S.make_workload(count, steps)
     S.run_batched(S.make_workload(2, 10, size_seed=7), budget)  # warm-up
-    S.run_sequential(S.make_workload(2, 10, size_seed=7))
+    S.run_sequential(S.make_workload(2, 10, size_seed=7), path="v1")
+    plain = S.run_sequential(reqs, path="v1")
     seq = S.run_sequential(reqs)
-    bat = S.run_batched(reqs, budget, policy="shortest", min_cells=16 << 20)
-    rate = load * count / seq["makespan_s"]
+    bat = S.run_batched(reqs, budget, policy="shortest", min_cells=min_cells)
+    rate = load * count / plain["makespan_s"]
     on = S.make_workload(count, steps, rate=rate)
-    seq_on = S.run_sequential(on, online=True)
-    bat_on = S.run_batched(on, budget, online=True, policy="shortest", min_cells=16 << 20)
+    res_on = {"plain": S.run_sequential(on, online=True, path="v1"), "engine_loop": S.run_sequential(on, online=True),
+              "batched": S.run_batched(on, budget, online=True, policy="shortest", min_cells=min_cells)}
     r3 = lambda x: round(float(x), 3)  # noqa: E731
     return {
         "requests": count, "steps": steps, "cell_steps": int(seq["cell_steps"]),
-        "sequential_mlups": round(seq["throughput_mlups"], 1), "batched_mlups": round(bat["throughput_mlups"], 1),
-        "throughput_ratio": r3(bat["throughput_mlups"] / seq["throughput_mlups"]),
+        "mlups": {"plain": round(plain["throughput_mlups"], 1), "engine_loop": round(seq["throughput_mlups"], 1),
+                  "batched": round(bat["throughput_mlups"], 1)},
+        "throughput_ratio_vs_plain": r3(bat["throughput_mlups"] / plain["throughput_mlups"]),
+        "throughput_ratio_vs_engine_loop": r3(bat["throughput_mlups"] / seq["throughput_mlups"]),
+        "offline_p50_s": {"plain": r3(plain["latency_p50_s"]), "batched": r3(bat["latency_p50_s"])},
         "online_rate_per_s": r3(rate),
-        "p50_s": {"sequential": r3(seq_on["latency_p50_s"]), "batched": r3(bat_on["latency_p50_s"])},
-        "p99_s": {"sequential": r3(seq_on["latency_p99_s"]), "batched": r3(bat_on["latency_p99_s"])},
+        "p50_s": {k: r3(v["latency_p50_s"]) for k, v in res_on.items()},
+        "p99_s": {k: r3(v["latency_p99_s"]) for k, v in res_on.items()},
         "unstable_requests": len(seq["unstable"]),
-        "policy": "shortest remaining work first, >= 16M cells per step",
+        "plain": "reference-compatible ABI, the reference's loop (bcond + col_mc per step), FP16 requests in FP32",
+        "policy": f"shortest remaining work first, >= {min_cells} cells per step",
     }
 
 

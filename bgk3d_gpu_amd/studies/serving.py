@@ -66,7 +66,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--budget-gb", type=float, default=32.0)
     ap.add_argument("--chunk", type=int, default=100)
-    ap.add_argument("--min-cells", type=int, default=16 << 20)
+    ap.add_argument("--min-cells", type=int, default=4 << 20)
     ap.add_argument("--load", type=float, default=0.8)
     ap.add_argument("--no-online", action="store_true")
     ap.add_argument("--no-per-size", action="store_true")
