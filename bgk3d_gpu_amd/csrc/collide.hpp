@@ -285,10 +285,10 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     else
         bgk_collide_ref<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
 
-    // stochastic rounding: one hash per cell and step, then one LCG step per
-    // store (deterministic: independent of scheduling)
+    // stochastic rounding: one hash per cell and step, then a Weyl sequence
+    // over the cell's stores (deterministic: independent of scheduling)
     uint32_t r = 0;
-    if (SR) r = hash32(seed ^ ((uint32_t)(row + i) * 0x9e3779b1u));
+    if (SR) r = hash32(seed ^ (uint32_t)(row + i));
 #pragma unroll
     for (int d = 0; d < 19; ++d) {
         S* dst;
@@ -296,8 +296,7 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
             dst = (d < 18 ? p.b[d] : p.a[18]) + (row + i);
         else
             dst = reinterpret_cast<S*>(reinterpret_cast<char*>(d < 18 ? p.b[d] : p.a[18]) + row * ES + (uint64_t)vi);
-        if (SR) r = lcg_next(r);
-        store_nt(dst, Rounder<S, SR>::cvt(o[d], r));
+        store_nt(dst, Rounder<S, SR>::cvt(o[d], SR ? sr_word(r, d) : 0u));
     }
 }
 

@@ -2,14 +2,17 @@
 // gfx950, boundaries folded into the fused pull, FP32/FP64/FP16/BF16
 // storage.
 //
-// Layout of one lattice: 19 (or 18) arrays in one allocation.  Each array is
-// the reference's (0:nx-1, 0:ny-1, 0:nz-1) halo array with nx rounded up to a
-// multiple of A = 128 bytes / sizeof(storage) and the array start shifted by
-// A-1 elements, so the first interior cell of every row (i = 1) begins a
-// 128-byte line: a wave's loads of the unshifted directions and all its
-// stores then touch whole lines.  Direction 19 is updated in place and lives
-// only in lattice 0; directions 1..18 alternate between the two lattices
-// (the reference's pointer swap, SRC/col_MC.F90:539-596), tracked by `cur`.
+// Layout of one lattice: the reference's (0:nx-1, 0:ny-1, 0:nz-1) halo
+// array per population, with nx rounded up to a multiple of A = 128 bytes /
+// sizeof(storage) and the start shifted by A-1 elements, so the first
+// interior cell of every row (i = 1) begins a 128-byte line: a wave's loads
+// of the unshifted directions and all its stores then touch whole lines.
+// The populations' z planes are interleaved (plane k of population d at
+// (19 k + d) planes from the start), so one step's reads and writes move
+// through two compact windows of the lattices instead of 37 separate
+// arrays.  Direction 19 is updated in place and lives only in lattice 0;
+// directions 1..18 alternate between the two lattices (the reference's
+// pointer swap, SRC/col_MC.F90:539-596), tracked by `cur`.
 #include "engine_internal.hpp"
 
 #include <algorithm>
@@ -57,11 +60,16 @@ namespace {
 // kernels specific to the engine
 // ---------------------------------------------------------------------------
 
+// `src` holds `n` values: consecutive z planes of `plane` values each; plane
+// q of them goes to dst + q * zstride (the lattice's z stride).
 template <typename S>
-__global__ void convert_in(const typename Storage<S>::compute* __restrict__ src, S* __restrict__ dst, long n)
+__global__ void convert_in(const typename Storage<S>::compute* __restrict__ src, S* __restrict__ dst, long n,
+                           long plane, long zstride)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) store_value(dst + t, Rounder<S, 0>::cvt(src[t], 0u));
+    if (t >= n) return;
+    const long q = t / plane;
+    store_value(dst + (q * zstride + (t - q * plane)), Rounder<S, 0>::cvt(src[t], 0u));
 }
 
 // Inflow halo x = 0, directions 1..5, every (j,k) incl. halo rows: the
@@ -196,9 +204,9 @@ int upload_state(bgk_handle& h)
     const bgk_config& c = h.cfg;
     // rest state: shifted populations are all zero, which the memset left
     if (c.init == 1 || c.init == 2) {
-        // Taylor-Green: host f_eq slab by slab (<= 16 planes of all 19 arrays)
+        // Taylor-Green: host f_eq slab by slab (<= 16 planes of all 19 populations)
         const int slab = 16;
-        const long plane = h.nxy;
+        const long plane = h.plane;
         std::vector<std::vector<C>> host(19, std::vector<C>((size_t)(plane * slab), C(0)));
         C* ptrs[19];
         for (int d = 0; d < 19; ++d) ptrs[d] = host[d].data();
@@ -221,7 +229,7 @@ int upload_state(bgk_handle& h)
                 for (int buf = 0; buf < 2; ++buf) {
                     if (buf == 1 && d == 18) continue;
                     hipLaunchKernelGGL(convert_in<S>, dim3(ceil_div(cnt, 256)), dim3(256), 0, h.stream, tmp,
-                                       h.arr<S>(buf, d) + plane * k0, cnt);
+                                       h.arr<S>(buf, d) + h.nxy * k0, cnt, plane, h.nxy);
                 }
                 if (hipStreamSynchronize(h.stream) != hipSuccess) rc = fail(BGK_EHIP, "conversion failed");
             }
@@ -362,9 +370,10 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
     h.nx = (int)round_up(c.l + 2, h.A);
     h.ny = c.m + 2;
     h.nz = c.n + 2;
-    h.nxy = (long)h.nx * h.ny;
+    h.plane = (long)h.nx * h.ny;
+    h.nxy = 19 * h.plane;
     h.base = h.A - 1;
-    h.astride = round_up(h.A + h.nxy * h.nz, h.A);
+    h.lstride = round_up(h.A + h.nxy * h.nz, h.A);
     // derived parameters in the collision kind (SRC/hencol.F90:31, :52;
     // SRC/bcond.driven.F90:61; SRC/build.bcond.F90:29-35)
     const bool dbl = c.storage == BGK_F64;

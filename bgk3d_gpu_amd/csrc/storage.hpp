@@ -67,10 +67,15 @@ __host__ __device__ inline uint32_t hash32(uint32_t x)
     return x;
 }
 
+// Random word of store d (0..18) of a cell whose hash is r0: a Weyl sequence
+// with step 2^32 / golden ratio.  Every word is uniform when r0 is, and the
+// high bits of one cell's 19 words are spread evenly over [0, 2^32): one
+// integer add per store (a multiplicative generator costs a quarter-rate
+// 32-bit multiply per store).
+__device__ __forceinline__ uint32_t sr_word(uint32_t r0, int d) { return r0 + (uint32_t)(d + 1) * 0x9e3779b9u; }
+
 // Round a float to the storage type.  SR = 0: nearest-even; SR = 1:
-// stochastic with the 32-bit random word r, one LCG step per store (its
-// high bits are the strong ones).
-__device__ __forceinline__ uint32_t lcg_next(uint32_t r) { return r * 747796405u + 2891336453u; }
+// stochastic with the 32-bit random word r.
 template <typename S, int SR>
 struct Rounder;
 
@@ -88,8 +93,9 @@ struct Rounder<f16_t, 0> {
 };
 // Stochastic rounding uses gfx950's conversions with a random operand
 // (v_cvt_sr_f16_f32, v_cvt_sr_bf16_f32): one instruction per store.  The
-// word handed over is r ^ (r >> 16), so both halves carry the well-mixed
-// high bits of the LCG stream whichever bits the conversion consumes.
+// conversion draws on the high bits of the word (with a word below 2^16 it
+// never rounds up; with k << 16, k uniform, it rounds 1 + 0.72 ulp up in
+// 72.0 % of cases for FP16 and BF16).
 typedef _Float16 sr_half2 __attribute__((ext_vector_type(2)));
 typedef __bf16 sr_bf162 __attribute__((ext_vector_type(2)));
 
@@ -98,7 +104,7 @@ struct Rounder<f16_t, 1> {
     __device__ __forceinline__ static f16_t cvt(float v, uint32_t r)
     {
         const sr_half2 z = {0, 0};
-        return __builtin_amdgcn_cvt_sr_f16_f32(z, v, r ^ (r >> 16), false)[0];
+        return __builtin_amdgcn_cvt_sr_f16_f32(z, v, r, false)[0];
     }
 };
 template <>
@@ -114,7 +120,7 @@ struct Rounder<bf16_t, 1> {
     __device__ __forceinline__ static bf16_t cvt(float v, uint32_t r)
     {
         const sr_bf162 z = {0, 0};
-        const __bf16 h = __builtin_amdgcn_cvt_sr_bf16_f32(z, v, r ^ (r >> 16), false)[0];
+        const __bf16 h = __builtin_amdgcn_cvt_sr_bf16_f32(z, v, r, false)[0];
         return bf16_t{__builtin_bit_cast(uint16_t, h)};
     }
 };

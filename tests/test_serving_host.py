@@ -7,13 +7,14 @@ from bgk3d_gpu_amd import serving as S
 
 
 def layout_bytes(n, esize):
-    # rows padded to 128-byte lines, array start shifted so i = 1 opens a line,
-    # 19 + 18 arrays, request region rounded to 64 KiB
+    # rows padded to 128-byte lines, start shifted so i = 1 opens a line,
+    # 19 population planes per z level in each of the two lattices, request
+    # region rounded to 64 KiB
     A = 128 // esize
     nx = -(-(n + 2) // A) * A
-    nxy = nx * (n + 2)
-    astride = -(-(A + nxy * (n + 2)) // A) * A
-    b = astride * 37 * esize
+    plane = nx * (n + 2)
+    lstride = -(-(A + 19 * plane * (n + 2)) // A) * A
+    b = lstride * 2 * esize
     return -(-b // 65536) * 65536
 
 
