@@ -27,10 +27,15 @@
 // on the `align`-element boundary at or before i = 1 (computed from the
 // output address), so that every wave's loads and stores of the same row
 // hit whole 128-byte lines whatever layout the caller chose; the engine's own
-// layout puts i = 1 on that boundary, making the shift zero.  Loads are plain
-// (neighbouring waves share the lines of the +-1 shifted pulls through L2);
-// stores are non-temporal (the output is not read again this step).
+// layout puts i = 1 on that boundary, making the shift zero.  Stores and the
+// loads of the directions with c_x = 0 are non-temporal (streaming: read or
+// written once per step); the +-x shifted loads keep the default policy, so
+// the line two neighbouring waves of a row share is found in L2 by the
+// second.
 #pragma once
+
+#include <type_traits>
+#include <utility>
 
 #include "common.hpp"
 #include "d3q19.hpp"
@@ -75,6 +80,33 @@ __device__ __forceinline__ void axis_fix(int cd, int s, int lo_hi_n, int& shift,
     }
 }
 
+// Calls f(std::integral_constant<int, d>) for d = 0..18: the direction is a
+// compile-time constant in the body, so per-direction choices (the load's
+// cache policy below) cannot be merged across directions by the compiler.
+template <typename F, int... D>
+__device__ __forceinline__ void for_each_dir_(F&& f, std::integer_sequence<int, D...>)
+{
+    (f(std::integral_constant<int, D>{}), ...);
+}
+template <typename F>
+__device__ __forceinline__ void for_each_dir(F&& f)
+{
+    for_each_dir_(f, std::make_integer_sequence<int, 19>{});
+}
+
+// Load of one pulled population.  Directions with c_x = 0 read each of
+// their lines exactly once in a step (streaming: non-temporal); the +-x
+// shifted directions share a line between neighbouring waves of a row, so
+// they use the default policy and the second wave finds it in L2.
+template <typename S, int CX>
+__device__ __forceinline__ typename Storage<S>::compute pull_load(const S* ptr)
+{
+    if constexpr (CX == 0)
+        return Storage<S>::load_nt(ptr);
+    else
+        return Storage<S>::load(ptr);
+}
+
 // EDGE = the row (j,k) touches a y or z boundary; otherwise only the x
 // direction can leave the interior and the j/k checks are compiled out.
 //
@@ -101,9 +133,9 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
         if (ri < (unsigned)p.oni && rj < (unsigned)p.onj && rk < (unsigned)p.onk)
             ob = p.obits[(long)ri + (long)p.oni * ((long)rj + (long)p.onj * rk)];
     }
-#pragma unroll
-    for (int d = 0; d < 19; ++d) {
-        const int cx = cx_of(d), cy = cy_of(d), cz = cz_of(d);
+    for_each_dir([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        constexpr int cx = cx_of(d), cy = cy_of(d), cz = cz_of(d);
         long ds = -((long)p.nx * cy + p.nxy * cz);  // scalar element shift (y, z)
         int dx = -cx;                                // per-lane element shift (x)
         bool bounce = false, wall = false, top = false;
@@ -130,26 +162,26 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
             const long s = row + i;
             const long src = s + ds + dx;
             const bool b = wall || bounce;
-            v = Storage<S>::load(b ? (p.a[opp_of(d)] + s) : (p.a[d] + src));
+            v = pull_load<S, cx>(b ? (p.a[opp_of(d)] + s) : (p.a[d] + src));
             if (BZ == AX_LID && EDGE) {
                 if (d == 1 && top) v = v + p.lidforce;
                 if (d == 10 && top) v = v - p.lidforce;
             }
             x[d] = v;
-            continue;
+            return;
         }
         const char* own = reinterpret_cast<const char*>(p.a[opp_of(d)]) + rowb;
         if (wall) {  // y or z wall: the whole wave bounces
-            v = Storage<S>::load(reinterpret_cast<const S*>(own + (uint64_t)vi));
+            v = pull_load<S, cx>(reinterpret_cast<const S*>(own + (uint64_t)vi));
         } else {
             const char* base = reinterpret_cast<const char*>(p.a[d]) + rowb + ds * ES;
             const S* nptr = reinterpret_cast<const S*>(base + (uint64_t)(vi + (uint32_t)dx * (uint32_t)ES));
             if ((BX == AX_WALL && cx != 0) || (OBST && d < 18)) {
                 // a lane may bounce: pick its address, one load
                 const S* optr = reinterpret_cast<const S*>(own + (uint64_t)vi);
-                v = Storage<S>::load(bounce ? optr : nptr);
+                v = pull_load<S, cx>(bounce ? optr : nptr);
             } else {
-                v = Storage<S>::load(nptr);
+                v = pull_load<S, cx>(nptr);
             }
         }
         if (BZ == AX_LID && EDGE) {
@@ -157,7 +189,7 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
             if (d == 10 && top) v = v - p.lidforce;
         }
         x[d] = v;
-    }
+    });
 }
 
 // Work mapping.  wpr = waves needed per (possibly shifted) row.  A block
@@ -279,6 +311,18 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     constexpr bool LES = (FMODE & 4) != 0;
     C fx, fy, fz;
     force_terms<C, (FMODE & 3)>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
+    if constexpr (sizeof(S) == 8 && !LES) {
+        // FP64: reference-order arithmetic, each value stored as soon as it
+        // is formed (77 vector registers instead of 86: 6 waves per SIMD
+        // without spilling).  The FP32 form keeps the array collide: with
+        // its stores spread through the arithmetic, rows that end in a
+        // partly interior line (n = 134 ... 182) ran 18-24 % slower.
+        bgk_collide_stream(x, p.omega, fx, fy, fz, p.cst, [&](auto dc, C v) {
+            constexpr int d = decltype(dc)::value;
+            store_nt((d < 18 ? p.b[d] : p.a[18]) + (row + i), v);
+        });
+        return;
+    }
     C o[19];
     if constexpr (sizeof(S) == 2)
         bgk_collide_lean<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
@@ -300,8 +344,20 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     }
 }
 
+// Waves per SIMD the compiler is asked to fit (1: no request).  FP64
+// without LES (streaming collide, 77 vector registers) is asked for 6; the
+// other forms are left to the compiler: FP32 reaches 8 by itself, the LES
+// forms need every equilibrium before relaxing, and the 16-bit forms ran
+// slower when pushed from 7 to 8 waves.
+template <typename S, int FMODE>
+struct FusedWaves {
+    static constexpr bool LES = (FMODE & 4) != 0;
+    static constexpr int value = (!LES && sizeof(S) == 8) ? 6 : 1;
+};
+
 template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
-__global__ __launch_bounds__(512) void collide_fused(FusedArgs<S> p, RowMap rm)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FusedWaves<S, FMODE>::value))) void collide_fused(
+    FusedArgs<S> p, RowMap rm)
 {
     fused_rows<S, BX, BY, BZ, OBST, FMODE, SR>(p, rm, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, p.seed);
 }

@@ -14,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace bgk {
 
 // c_x, c_y, c_z of directions 1..19
@@ -184,6 +186,92 @@ __device__ __forceinline__ void bgk_collide_ref(const R (&x)[19], R omega, R fx,
     o[16] = x[16] - omega * (x[16] - e[16]) - fy;
     o[17] = x[17] - omega * (x[17] - e[17]) - fy + fz;
     o[18] = x[18] - omega * (x[18] - e[18]);
+}
+
+// The same BGK step as bgk_collide_ref, handing each post-collision value to
+// sink(std::integral_constant<int, d>{}, value) as soon as it is formed.
+// Every value is computed with exactly the expression of bgk_collide_ref
+// (same operands, same order, same roundings), so the results are identical;
+// only the order in which the 19 values are produced differs: pairs of
+// opposite directions share their quadratic term, and a population's
+// register is free once its value has been emitted.  Without LES (whose
+// relaxation rate needs every equilibrium first) this keeps far fewer values
+// live than the array form.
+template <int D>
+using DirC = std::integral_constant<int, D>;
+
+template <typename R, typename Sink>
+__device__ __forceinline__ void bgk_collide_stream(const R (&x)[19], R omega, R fx, R fy, R fz,
+                                                   const CollideConsts<R>& c, Sink&& sink)
+{
+    const R uno = R(1), cte1 = c.cte1, cte0 = c.cte0;
+    const R rf = c.rf, qf = c.qf, tre = c.tre;
+    const R p0 = c.p0, p1 = c.p1, p2 = c.p2;
+
+    const R rho = x[0] + x[1] + x[2] + x[3] + x[4] + x[5] + x[6] + x[7] + x[8] + x[9] + x[10] + x[11] +
+                  x[12] + x[13] + x[14] + x[15] + x[16] + x[17] + x[18] + cte1;
+    const R rhoinv = uno / rho;
+    const R vx = (x[0] + x[1] + x[2] + x[3] + x[4] - x[9] - x[10] - x[11] - x[12] - x[13]) * rhoinv;
+    const R vy = (x[2] + x[6] + x[7] + x[8] + x[11] - x[0] - x[9] - x[15] - x[16] - x[17]) * rhoinv;
+    const R vz = (x[3] + x[5] + x[6] + x[12] + x[17] - x[1] - x[8] - x[10] - x[14] - x[15]) * rhoinv;
+    const R vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+    const R vsq = vx2 + vy2 + vz2;
+    const R rp0 = rho * p0, rp1 = rho * p1, rp2 = rho * p2;
+    const R s2 = cte1 * (rp2 - p2), s1 = cte1 * (rp1 - p1), s0 = cte1 * (rp0 - p0);
+
+    // diagonal pair along v = (c_a +- c_b).u: q = cte0 + qf (tre v v - vsq), scaled rf v
+    auto qd = [&](R v) { return cte0 + qf * (tre * v * v - vsq); };
+    // axis pair: q = cte0 + qf (tre v^2 - vsq) with v^2 the squared component
+    auto qa = [&](R v2) { return cte0 + qf * (tre * v2 - vsq); };
+    auto rel = [&](R xd, R e) { return xd - omega * (xd - e); };
+
+    {  // x - y: directions 1 and 12
+        const R v = vx - vy, q = qd(v), r = rf * v;
+        sink(DirC<0>{}, rel(x[0], rp2 * (r + q) + s2) + fx - fy);
+        sink(DirC<11>{}, rel(x[11], rp2 * (-r + q) + s2) - fx + fy);
+    }
+    {  // x - z: directions 2 and 13
+        const R v = vx - vz, q = qd(v), r = rf * v;
+        sink(DirC<1>{}, rel(x[1], rp2 * (r + q) + s2) + fx - fz);
+        sink(DirC<12>{}, rel(x[12], rp2 * (-r + q) + s2) - fx + fz);
+    }
+    {  // x + y: directions 3 and 10
+        const R v = vx + vy, q = qd(v), r = rf * v;
+        sink(DirC<2>{}, rel(x[2], rp2 * (r + q) + s2) + fx + fy);
+        sink(DirC<9>{}, rel(x[9], rp2 * (-r + q) + s2) - fx - fy);
+    }
+    {  // x + z: directions 4 and 11
+        const R v = vx + vz, q = qd(v), r = rf * v;
+        sink(DirC<3>{}, rel(x[3], rp2 * (r + q) + s2) + fx + fz);
+        sink(DirC<10>{}, rel(x[10], rp2 * (-r + q) + s2) - fx - fz);
+    }
+    {  // x: directions 5 and 14
+        const R q = qa(vx2), r = rf * vx;
+        sink(DirC<4>{}, rel(x[4], rp1 * (r + q) + s1) + fx);
+        sink(DirC<13>{}, rel(x[13], rp1 * (-r + q) + s1) - fx);
+    }
+    {  // z: directions 6 and 15
+        const R q = qa(vz2), r = rf * vz;
+        sink(DirC<5>{}, rel(x[5], rp1 * (r + q) + s1) + fz);
+        sink(DirC<14>{}, rel(x[14], rp1 * (-r + q) + s1) - fz);
+    }
+    {  // y + z: directions 7 and 16
+        const R v = vy + vz, q = qd(v), r = rf * v;
+        sink(DirC<6>{}, rel(x[6], rp2 * (r + q) + s2) + fy + fz);
+        sink(DirC<15>{}, rel(x[15], rp2 * (-r + q) + s2) - fy - fz);
+    }
+    {  // y: directions 8 and 17
+        const R q = qa(vy2), r = rf * vy;
+        sink(DirC<7>{}, rel(x[7], rp1 * (r + q) + s1) + fy);
+        sink(DirC<16>{}, rel(x[16], rp1 * (-r + q) + s1) - fy);
+    }
+    {  // y - z: directions 9 and 18
+        const R v = vy - vz, q = qd(v), r = rf * v;
+        sink(DirC<8>{}, rel(x[8], rp2 * (r + q) + s2) + fy - fz);
+        sink(DirC<17>{}, rel(x[17], rp2 * (-r + q) + s2) - fy + fz);
+    }
+    // rest: direction 19
+    sink(DirC<18>{}, rel(x[18], rp0 * (cte0 + qf * (-vsq)) + s0));
 }
 
 // The same collision with FMA contraction allowed, for the 16-bit storage

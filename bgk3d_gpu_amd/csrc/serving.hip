@@ -38,7 +38,9 @@ using namespace bgk::eng;
 
 namespace {
 
-constexpr int kServeWaves = 4;  // block = 256 threads for every request
+// Block = 384 threads for every request: rows of 1, 2, 3 or 6 waves (up to
+// 384 cells) fill a block with no idle wave (6, 3, 2 or 1 rows per block).
+constexpr int kServeWaves = 6;
 constexpr size_t kArenaAlign = 64 << 10;
 
 template <typename S>
@@ -55,8 +57,8 @@ struct BatchSlot {
 // One time step of every request of one storage group.  The lid-driven
 // cavity: bounce-back walls in x and y, lid at z = n + 1 (SRC/bcond.driven.F90).
 template <typename S, int SR>
-__global__ __launch_bounds__(64 * kServeWaves) void collide_batched(const BatchSlot<S>* __restrict__ slots,
-                                                                    const uint32_t* __restrict__ owner, int t)
+__global__ __launch_bounds__(64 * kServeWaves) __attribute__((amdgpu_waves_per_eu(FusedWaves<S, 0>::value))) void
+collide_batched(const BatchSlot<S>* __restrict__ slots, const uint32_t* __restrict__ owner, int t)
 {
     const uint32_t b = blockIdx.x;
     const BatchSlot<S>& sl = slots[owner[b]];

@@ -8,8 +8,9 @@
 // Stored values are the reference's shifted populations f_i - w_i
 // (SRC/col_MC.F90:139-144), which keeps them near zero where low-precision
 // formats have the most resolution.  Low-precision stores round to nearest
-// even, or stochastically (unbiased) with a counter-based hash so that a run
-// is reproducible and independent of scheduling.
+// even, or stochastically (unbiased) with counter-based random words so that
+// a run is reproducible and independent of scheduling.  load_nt is the
+// streaming form used by the time step (each value is read once per step).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,18 +34,21 @@ struct Storage<float> {
     typedef float compute;
     static constexpr int kind = ST_F32;
     __device__ __forceinline__ static float load(const float* p) { return *p; }
+    __device__ __forceinline__ static float load_nt(const float* p) { return __builtin_nontemporal_load(p); }
 };
 template <>
 struct Storage<double> {
     typedef double compute;
     static constexpr int kind = ST_F64;
     __device__ __forceinline__ static double load(const double* p) { return *p; }
+    __device__ __forceinline__ static double load_nt(const double* p) { return __builtin_nontemporal_load(p); }
 };
 template <>
 struct Storage<f16_t> {
     typedef float compute;
     static constexpr int kind = ST_F16;
     __device__ __forceinline__ static float load(const f16_t* p) { return (float)*p; }
+    __device__ __forceinline__ static float load_nt(const f16_t* p) { return (float)__builtin_nontemporal_load(p); }
 };
 template <>
 struct Storage<bf16_t> {
@@ -53,6 +57,10 @@ struct Storage<bf16_t> {
     __device__ __forceinline__ static float load(const bf16_t* p)
     {
         return __uint_as_float((uint32_t)p->u << 16);
+    }
+    __device__ __forceinline__ static float load_nt(const bf16_t* p)
+    {
+        return __uint_as_float((uint32_t)__builtin_nontemporal_load(&p->u) << 16);
     }
 };
 

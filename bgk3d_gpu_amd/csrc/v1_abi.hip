@@ -428,14 +428,14 @@ int bgk_device_count(int* n)
 
 // ---- measured stream bandwidth: copy of n floats (n*8 bytes moved) -------
 typedef float v4f __attribute__((ext_vector_type(4)));
-// plain loads and non-temporal stores, the same memory instructions as the
-// fused collide, so its counters calibrate the collide's.  One 16-byte
-// element per thread over the whole buffer: on gfx950 this runs about 30 %
-// faster than a grid-stride loop over a capped grid.
+// non-temporal loads and stores, the same memory instructions as the fused
+// collide, so its counters calibrate the collide's.  One 16-byte element per
+// thread over the whole buffer: on gfx950 this runs about 30 % faster than a
+// grid-stride loop over a capped grid.
 __global__ void stream_copy_kernel(const v4f* __restrict__ src, v4f* __restrict__ dst, long n4)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n4) __builtin_nontemporal_store(src[i], dst + i);
+    if (i < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 int bgk_stream_copy(const void* src, void* dst, size_t n_floats)
 {
