@@ -7,7 +7,7 @@ and z, Re = 300 on the diameter (nu = 0.05 * 64 / 300), rest start.  Drag
 and lift by momentum exchange with FP64 accumulation every 100 steps
 (SRC/draglift.F90), reported both in the reference's normalisation
 F / (u_inflow^2 R) and as a drag coefficient C = F / (0.5 u^2 pi R^2).
-Memory: two lattices of 37 arrays, 23.3 GB.
+Memory: two lattices of 19 population planes per z level, 23.9 GB.
 
     python -m bgk3d_gpu_amd.studies.sphere [--steps N] [--every K] > out.json
 """
