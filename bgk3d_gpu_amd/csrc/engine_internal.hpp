@@ -166,6 +166,7 @@ FusedArgs<S> fused_args(const bgk_handle& h, int cur)
     // pull the cells after i = l up to that line's end are never read
     p.lend = h.cfg.boundary == BGK_BC_INFLOW ? 0 : (int)round_up(h.cfg.l, h.A);
     p.seed = step_seed(h.cfg.seed, h.steps);
+    p.pstride = h.plane;
     return p;
 }
 
