@@ -42,6 +42,13 @@ namespace {
 // 384 cells) fill a block with no idle wave (6, 3, 2 or 1 rows per block).
 constexpr int kServeWaves = 6;
 constexpr size_t kArenaAlign = 64 << 10;
+// A chunk runs its steps sub-batch by sub-batch: consecutive requests of a
+// storage group whose lattices add up to at most this many bytes (one larger
+// request forms its own).  The fused kernel streams about 6 % faster over a
+// 1.5-3 GB working set than over 10-20 GB, whose address translations do not
+// stay cached from one launch to the next; every request still advances all
+// the chunk's steps in order, so results do not depend on the grouping.
+constexpr size_t kSubBatchBytes = size_t(3) << 30;
 
 template <typename S>
 struct BatchSlot {
@@ -58,9 +65,9 @@ struct BatchSlot {
 // cavity: bounce-back walls in x and y, lid at z = n + 1 (SRC/bcond.driven.F90).
 template <typename S, int SR>
 __global__ __launch_bounds__(64 * kServeWaves) __attribute__((amdgpu_waves_per_eu(FusedWaves<S, 0>::value))) void
-collide_batched(const BatchSlot<S>* __restrict__ slots, const uint32_t* __restrict__ owner, int t)
+collide_batched(const BatchSlot<S>* __restrict__ slots, const uint32_t* __restrict__ owner, uint32_t base, int t)
 {
-    const uint32_t b = blockIdx.x;
+    const uint32_t b = base + blockIdx.x;
     const BatchSlot<S>& sl = slots[owner[b]];
     const RowMap rm = sl.rm;
     const uint32_t lb = b - sl.block0;
@@ -212,13 +219,25 @@ int grow_group(bgks_server& s, bgks_server::Group& g, size_t slot_bytes, size_t 
 // Build and upload the launch tables of one storage group: a slot per
 // request (kernel arguments for both lattice parities, row mapping, seed,
 // step count at the chunk start) and the owner of every block.
+struct SubBatch {
+    uint32_t block0, nblocks;
+};
+
 template <typename S>
-int prepare_group(bgks_server& s, bgks_server::Group& g, const std::vector<Instance*>& members, uint32_t& nblocks)
+int prepare_group(bgks_server& s, bgks_server::Group& g, const std::vector<Instance*>& members,
+                  std::vector<SubBatch>& subs)
 {
     std::vector<BatchSlot<S>> slots(members.size());
     uint32_t total = 0;
+    size_t sub_bytes = 0;
+    subs.clear();
     for (size_t q = 0; q < members.size(); ++q) {
         bgk_handle& h = members[q]->h;
+        if (subs.empty() || (sub_bytes > 0 && sub_bytes + members[q]->len > kSubBatchBytes)) {
+            subs.push_back({total, 0});
+            sub_bytes = 0;
+        }
+        sub_bytes += members[q]->len;
         BatchSlot<S>& sl = slots[q];
         sl.args[0] = fused_args<S>(h, 0);
         sl.args[1] = fused_args<S>(h, 1);
@@ -230,7 +249,10 @@ int prepare_group(bgks_server& s, bgks_server::Group& g, const std::vector<Insta
         sl.block0 = total;
         const long long nb = (long long)sl.rm.bpr * sl.gy * h.cfg.n;
         if ((long long)total + nb > 0xffffffffLL) return fail(BGK_EINVAL, "batch grid exceeds 2^32 blocks");
+        if ((long long)subs.back().nblocks + nb > 0x7fffffffLL)
+            return fail(BGK_EINVAL, "sub-batch grid exceeds 2^31 blocks");
         total += (uint32_t)nb;
+        subs.back().nblocks += (uint32_t)nb;
     }
     if (int rc = grow_group(s, g, slots.size() * sizeof(BatchSlot<S>), total)) return rc;
     std::memcpy(g.h_slots, slots.data(), slots.size() * sizeof(BatchSlot<S>));
@@ -242,15 +264,14 @@ int prepare_group(bgks_server& s, bgks_server::Group& g, const std::vector<Insta
     ENG_HIP(hipMemcpyAsync(g.d_slots, g.h_slots, slots.size() * sizeof(BatchSlot<S>), hipMemcpyHostToDevice,
                            s.stream));
     ENG_HIP(hipMemcpyAsync(g.d_owner, g.h_owner, (size_t)total * sizeof(uint32_t), hipMemcpyHostToDevice, s.stream));
-    nblocks = total;
     return 0;
 }
 
 template <typename S, int SR>
-void launch_group(bgks_server& s, bgks_server::Group& g, uint32_t nblocks, int t)
+void launch_group(bgks_server& s, bgks_server::Group& g, const SubBatch& sb, int t)
 {
-    hipLaunchKernelGGL((collide_batched<S, SR>), dim3(nblocks), dim3(64 * kServeWaves), 0, s.stream,
-                       static_cast<const BatchSlot<S>*>(g.d_slots), g.d_owner, t);
+    hipLaunchKernelGGL((collide_batched<S, SR>), dim3(sb.nblocks), dim3(64 * kServeWaves), 0, s.stream,
+                       static_cast<const BatchSlot<S>*>(g.d_slots), g.d_owner, sb.block0, t);
 }
 
 template <typename F>
@@ -448,23 +469,23 @@ int bgks_advance(bgks_server* s, long long* done_ids, int max_done, int* n_done)
     if (K > 0 && !active.empty()) {
         std::vector<Instance*> members[8];
         for (Instance* in : active) members[group_of(in->h.cfg.storage, in->h.cfg.rounding)].push_back(in);
-        uint32_t nblocks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        std::vector<SubBatch> subs[8];
         for (int g = 0; g < 8; ++g) {
             if (members[g].empty()) continue;
             int rc = by_group(g, [&](auto tag, auto) {
-                return prepare_group<decltype(tag)>(*s, s->groups[g], members[g], nblocks[g]);
+                return prepare_group<decltype(tag)>(*s, s->groups[g], members[g], subs[g]);
             });
             if (rc) return rc;
         }
-        for (long long t = 0; t < K; ++t)
-            for (int g = 0; g < 8; ++g) {
-                if (!nblocks[g]) continue;
-                by_group(g, [&](auto tag, auto sr) {
-                    launch_group<decltype(tag), decltype(sr)::value>(*s, s->groups[g], nblocks[g], (int)t);
-                    return 0;
-                });
-                ++s->launches;
-            }
+        for (int g = 0; g < 8; ++g)
+            for (const SubBatch& sb : subs[g])
+                for (long long t = 0; t < K; ++t) {
+                    by_group(g, [&](auto tag, auto sr) {
+                        launch_group<decltype(tag), decltype(sr)::value>(*s, s->groups[g], sb, (int)t);
+                        return 0;
+                    });
+                    ++s->launches;
+                }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(BGK_EHIP, "batched step launch: %s", hipGetErrorString(e));
         for (Instance* in : active) {

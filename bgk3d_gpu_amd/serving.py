@@ -1,9 +1,9 @@
 """Serving many lid-driven cavity solves on one GPU.
 
 `Server` binds the continuous-batching server of include/bgk3d_engine.h
-(bgks_*): requests are admitted in arrival order within an HBM budget and
-every active request advances with one kernel launch per storage group and
-step.  `run_batched` drives it over a request stream; `run_sequential` is
+(bgks_*): requests are admitted within an HBM budget (arrival order, or
+shortest remaining work first) and the scheduled requests advance with one
+kernel launch per storage group, step and sub-batch of at most 3 GB.  `run_batched` drives it over a request stream; `run_sequential` is
 the plain loop the reference's sweep script stands for
 (UTIL/run_sweep.sh:55-63, 128-169): one solve after the other, each with its
 own handle (allocate, rest start, `steps` time steps, diagnostics, free).

@@ -145,9 +145,9 @@ const char* bgk_last_error(void);
  *
  * Requests are admitted while the device arena reserved by bgks_create (the
  * HBM budget) can hold their two lattices; the requests scheduled for a
- * chunk advance by one kernel launch per (storage, rounding) group and
- * step; requests join and leave between chunks of at most chunk_steps
- * steps (see enum bgks_policy).  A request's populations are bit-identical
+ * chunk advance by one kernel launch per (storage, rounding) group, step
+ * and sub-batch of at most 3 GB of lattices; requests join and leave
+ * between chunks of at most chunk_steps steps (see enum bgks_policy).  A request's populations are bit-identical
  * to those of a handle (bgk_create with the same n, svisc, u0, storage,
  * rounding and seed) after the same number of steps, whatever else is
  * running.  Not thread-safe: one caller drives a server.
