@@ -4,7 +4,8 @@
   three waves per row, FP32 / FP64 / FP16 / BF16, nearest and stochastic
   rounding, different step counts so requests join and leave mid-run) ends
   with populations bit-identical to the same request run alone on an engine
-  handle, and to the same batch run with another chunk length.
+  handle, and to the same batch run with another chunk length; also when
+  a chunk runs in several sub-batches (more than 3 GB of lattices).
 * Admission: with a budget that holds two of the largest requests, no more
   than the budget is ever in use, requests start in arrival order and all finish.
 * The diagnostics the server reports equal bgk_diagno of the alone run.
@@ -73,6 +74,26 @@ def test_batched_requests_match_alone_runs(gpu):
         assert not res[q]["unstable"]
         assert np.all(np.isfinite(p_alone))
         assert np.any(p_alone != 0.0)     # the lid moved the fluid
+
+
+def test_requests_across_sub_batches_match_alone_runs(gpu):
+    """Requests whose lattices add up to more than one 3 GB sub-batch of a
+    chunk (five FP32 cavities of 150^3 and 160^3, 3.4 GB) and two FP16
+    ones: the sub-batch split does not change any request's populations."""
+    spec = [(160, 900.0, "f32", 0, 23), (150, 300.0, "f32", 0, 31), (160, 2000.0, "f32", 0, 17),
+            (150, 700.0, "f32", 0, 29), (160, 1200.0, "f32", 0, 23), (140, 500.0, "f16", 1, 19),
+            (150, 800.0, "f16", 1, 27)]
+    reqs = [S.Request(n, re, st, steps, seed=40 + q, rounding=rnd, keep=True)
+            for q, (n, re, st, rnd, steps) in enumerate(spec)]
+    total = sum(S.request_bytes(r) for r in reqs if r.storage == "f32")
+    assert total > 3 << 30
+    banner(f"serving across sub-batches: {len(reqs)} requests, FP32 lattices {total / 2**30:.2f} GiB")
+    pops, res = serve(reqs, 10, budget=16 << 30)
+    for q, r in enumerate(reqs):
+        banner(f"alone run {q}: n={r.n} {r.storage} steps={r.steps}")
+        p_alone, d_alone = alone(r)
+        assert np.array_equal(pops[q], p_alone, equal_nan=True), f"request {q} differs from its alone run"
+        assert np.array_equal(res[q]["diag"], d_alone, equal_nan=True)
 
 
 def test_chunk_length_does_not_change_results(gpu):
