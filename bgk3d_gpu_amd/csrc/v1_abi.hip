@@ -61,6 +61,36 @@ Geometry geom(int l, int m, int n, int nx, int ny)
 
 constexpr int kFaceThreads = 256;
 
+// Host-side check of the caller's arguments before any kernel is launched on
+// them: positive sizes, rows and planes that hold the halo (nx >= l+2,
+// ny >= m+2), and non-null population pointers (the first `na` of `a`, the
+// first `nb` of `b`).  The reference performs no such check and faults on
+// the device; here the call is refused with a message on stderr (the
+// reference's way of reporting launch problems, SRC/col_MC_gpu.hpp:40-47).
+bool shape_ok(const char* who, int l, int m, int n, int nx, int ny, const void* const* a, int na,
+              const void* const* b = nullptr, int nb = 0)
+{
+    const char* why = nullptr;
+    if (l <= 0 || m <= 0 || n <= 0) why = "sizes must be positive";
+    else if (nx < l + 2 || ny < m + 2) why = "nx, ny must hold the halo (nx >= l+2, ny >= m+2)";
+    else if ((na > 0 && !a) || (nb > 0 && !b)) why = "null pointer table";
+    for (int d = 0; !why && d < na; ++d)
+        if (!a[d]) why = "null population pointer";
+    for (int d = 0; !why && d < nb; ++d)
+        if (!b[d]) why = "null population pointer";
+    if (why) {
+        std::fprintf(stderr, "bgk3d_gpu: %s(l=%d m=%d n=%d nx=%d ny=%d): %s; nothing launched\n", who, l, m, n, nx,
+                     ny, why);
+        std::fflush(stderr);
+        return false;
+    }
+    return true;
+}
+bool shape_ok(const char* who, int l, int m, int n, int nx, int ny, real_t* const* a)
+{
+    return shape_ok(who, l, m, n, nx, ny, reinterpret_cast<const void* const*>(a), 19);
+}
+
 template <typename K, typename... Args>
 void launch_face(K kern, long count, const char* name, Args... args)
 {
@@ -167,6 +197,9 @@ void col_mc_gpu(const real_t* a01, const real_t* a02, const real_t* a03, const r
     const real_t* in[19] = {a01, a02, a03, a04, a05, a06, a07, a08, a09, a10,
                             a11, a12, a13, a14, a15, a16, a17, a18, a19};
     real_t* out[18] = {b01, b02, b03, b04, b05, b06, b07, b08, b09, b10, b11, b12, b13, b14, b15, b16, b17, b18};
+    if (!shape_ok("col_mc_gpu", l, m, n, nx, ny, reinterpret_cast<const void* const*>(in), 19,
+                  reinterpret_cast<const void* const*>(out), 18))
+        return;
     Lattice19<real_t> A, B;
     for (int d = 0; d < 19; ++d) A.p[d] = const_cast<real_t*>(in[d]);
     for (int d = 0; d < 18; ++d) B.p[d] = out[d];
@@ -178,6 +211,9 @@ void col_mc_gpu(const real_t* a01, const real_t* a02, const real_t* a03, const r
 void col_mc_force_gpu(real_t** a, real_t** b, int l, int m, int n, int nx, int ny, real_t omega, real_t fgrad,
                       int axis)
 {
+    if (!shape_ok("col_mc_force_gpu", l, m, n, nx, ny, reinterpret_cast<const void* const*>(a), 19,
+                  reinterpret_cast<const void* const*>(b), 18))
+        return;
     Lattice19<real_t> A = lat_from(a), B = lat_from(b);
     B.p[18] = A.p[18];
     const Geometry g = geom(l, m, n, nx, ny);
@@ -193,6 +229,9 @@ void col_mc_force_gpu(real_t** a, real_t** b, int l, int m, int n, int nx, int n
 void col_mc_les_gpu(real_t** a, real_t** b, int l, int m, int n, int nx, int ny, real_t omega1, real_t cte_s,
                     real_t fgrad, int axis)
 {
+    if (!shape_ok("col_mc_les_gpu", l, m, n, nx, ny, reinterpret_cast<const void* const*>(a), 19,
+                  reinterpret_cast<const void* const*>(b), 18))
+        return;
     Lattice19<real_t> A = lat_from(a), B = lat_from(b);
     B.p[18] = A.p[18];
     const Geometry g = geom(l, m, n, nx, ny);
@@ -207,6 +246,7 @@ void col_mc_les_gpu(real_t** a, real_t** b, int l, int m, int n, int nx, int ny,
 
 void bcond_periodic_gpu(real_t** a, int l, int m, int n, int nx, int ny)
 {
+    if (!shape_ok("bcond_periodic_gpu", l, m, n, nx, ny, a)) return;
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
     launch_face(periodic_x<real_t>, (long)(m + 2) * (n + 2), "periodic_x", A, g);
@@ -216,6 +256,7 @@ void bcond_periodic_gpu(real_t** a, int l, int m, int n, int nx, int ny)
 
 void bcond_channel_gpu(real_t** a, int l, int m, int n, int nx, int ny)
 {
+    if (!shape_ok("bcond_channel_gpu", l, m, n, nx, ny, a)) return;
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
     launch_face(periodic_x<real_t>, (long)(m + 2) * (n + 2), "periodic_x", A, g);
@@ -225,6 +266,7 @@ void bcond_channel_gpu(real_t** a, int l, int m, int n, int nx, int ny)
 
 void bcond_driven_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t force)
 {
+    if (!shape_ok("bcond_driven_gpu", l, m, n, nx, ny, a)) return;
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
     launch_face(driven_x<real_t>, (long)m * n, "driven_x", A, g);
@@ -236,6 +278,7 @@ void bcond_inflow_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t u_
                       real_t tre, real_t p1, real_t p2, real_t cte1)
 {
     (void)tre;
+    if (!shape_ok("bcond_inflow_gpu", l, m, n, nx, ny, a)) return;
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
     launch_face(inflow_x<real_t>, (long)(m + 2) * (n + 2), "inflow_x", A, g, u_inflow, rf, qf, p1, p2, cte1);
@@ -246,12 +289,18 @@ void bcond_inflow_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t u_
 void bcond_obs_gpu(real_t** a, const int* obs, int l, int m, int n, int nx, int ny, int imin, int imax, int jmin,
                    int jmax, int kmin, int kmax, int obs_nx, int obs_ny)
 {
-    if (obs_nx != l || obs_ny != m) {
+    if (!shape_ok("bcond_obs_gpu", l, m, n, nx, ny, a)) return;
+    if (!obs || obs_nx != l || obs_ny != m) {
         std::fprintf(stderr, "bgk3d_gpu: bcond_obs_gpu expects the (1:l,1:m,1:n) mask (got %d x %d)\n", obs_nx,
                      obs_ny);
         return;
     }
     if (imax < imin || jmax < jmin || kmax < kmin) return;  // no solid cell
+    if (imin < 1 || imax > l || jmin < 1 || jmax > m || kmin < 1 || kmax > n) {
+        std::fprintf(stderr, "bgk3d_gpu: bcond_obs_gpu box [%d:%d,%d:%d,%d:%d] leaves (1:l,1:m,1:n)\n", imin,
+                     imax, jmin, jmax, kmin, kmax);
+        return;
+    }
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
     launch_face(obstacle_bb<real_t>, (long)(imax - imin + 1) * (jmax - jmin + 1) * (kmax - kmin + 1),
@@ -276,7 +325,7 @@ int bgk_set_stream(void* stream)
 
 int diagno_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t cte1, double* out)
 {
-    if (l <= 0 || m <= 0 || n <= 0) return -1;
+    if (!shape_ok("diagno_gpu", l, m, n, nx, ny, a) || !out) return -1;
     double* host = nullptr;
     double* dev = scratch((size_t)(kRedBlocks + 1) * 5, &host);
     if (!dev) return -2;
@@ -297,7 +346,7 @@ int diagno_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t cte1, dou
 int varm_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t cte1, double* u_k, double* w_k, double* v_k,
              double* den_j)
 {
-    if (l <= 0 || m <= 0 || n <= 0) return -1;
+    if (!shape_ok("varm_gpu", l, m, n, nx, ny, a) || !u_k || !w_k || !v_k || !den_j) return -1;
     double* host = nullptr;
     double* dev = scratch((size_t)3 * n + m, &host);
     if (!dev) return -2;
@@ -323,6 +372,7 @@ int varm_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t cte1, doubl
 int draglift_gpu(real_t** a, const int* obs, int l, int m, int n, int nx, int ny, int icoord, int jcoord,
                  int kcoord, int radius, real_t u_inflow, double* out)
 {
+    if (!shape_ok("draglift_gpu", l, m, n, nx, ny, a) || !obs || !out) return -1;
     // bounding box of SRC/draglift.F90:54-62 (delta = 3)
     const int delta = 3;
     const int is = icoord - radius - delta, ie = icoord + radius + delta;
@@ -358,7 +408,8 @@ static int line_profile(real_t** a, int l, int m, int n, int nx, int ny, int axi
 {
     const int len = axis == 0 ? l : (axis == 1 ? m : n);
     const int lim1 = axis == 0 ? m : l, lim2 = axis == 2 ? m : n;
-    if (l <= 0 || m <= 0 || n <= 0 || count <= 0 || first < 1 || first + count - 1 > len) return -1;
+    if (!shape_ok("line profile", l, m, n, nx, ny, a) || !out) return -1;
+    if (count <= 0 || first < 1 || first + count - 1 > len) return -1;
     if (c1 < 0 || c1 > lim1 + 1 || c2 < 0 || c2 > lim2 + 1) return -1;
     double* host = nullptr;
     const size_t bytes = (size_t)4 * count * sizeof(real_t);

@@ -116,6 +116,45 @@ def test_case_validation():
         Case("ldc", 0, 4, 4)
 
 
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+def test_reference_abi_refuses_bad_arguments(prec, capfd):
+    """Inconsistent sizes or null pointers are refused on the host before any
+    launch (the reference would fault on the device): the void entry points
+    print and return, the int ones return -1.  No GPU call is made."""
+    lib = rt.v1(prec)
+    R = rt.real_ctype(prec)
+    fake = (ctypes.c_void_p * 19)(*[0x10000 + 0x1000 * d for d in range(19)])
+    holed = (ctypes.c_void_p * 19)(*[0x10000 + 0x1000 * d if d != 7 else 0 for d in range(19)])
+    out = (ctypes.c_double * 5)()
+    l, m, n = 8, 6, 4
+    # rows too short for the halo
+    assert lib.diagno_gpu(fake, l, m, n, l + 1, m + 2, R(1), out) == -1
+    assert "must hold the halo" in capfd.readouterr().err
+    # a null population pointer
+    assert lib.diagno_gpu(holed, l, m, n, l + 2, m + 2, R(1), out) == -1
+    assert "null population pointer" in capfd.readouterr().err
+    # non-positive size
+    lib.bcond_periodic_gpu(fake, 0, m, n, l + 2, m + 2)
+    assert "sizes must be positive" in capfd.readouterr().err
+    lib.bcond_driven_gpu(fake, l, m, n, l + 2, m + 1, R(0.01))
+    assert "must hold the halo" in capfd.readouterr().err
+    lib.col_mc_force_gpu(fake, holed, l, m, n, l + 2, m + 2, R(1.2), R(0), 1)
+    assert "null population pointer" in capfd.readouterr().err
+    args = [ctypes.c_void_p(0x10000 + 0x1000 * d) for d in range(37)]
+    args[20] = ctypes.c_void_p(0)
+    lib.col_mc_gpu(*args, l, m, n, l + 2, m + 2, *[R(v) for v in (1.2, 0, 1, 1 / 3, 1 / 18, 1 / 36, 3, 1.5, 3,
+                                                                   0, 0, 0)])
+    assert "null population pointer" in capfd.readouterr().err
+    # obstacle box outside the lattice
+    mask = np.zeros((n, m, l), dtype=np.int32)
+    lib.bcond_obs_gpu(fake, mask.ctypes.data, l, m, n, l + 2, m + 2, 0, 3, 1, 3, 1, 3, l, m)
+    assert "leaves" in capfd.readouterr().err
+    u = (ctypes.c_double * n)()
+    assert lib.varm_gpu(fake, l, m, n, l + 2, m + 2, R(1), u, u, u, None) == -1
+    assert lib.prof_k_gpu(fake, l, m, n, l, m + 2, 2, 2, R(1), (R * (4 * n))()) == -1
+    capfd.readouterr()
+
+
 def test_no_cpu_fallback_without_gpu():
     from conftest import gpu_present
     if gpu_present():
