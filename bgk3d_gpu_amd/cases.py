@@ -38,6 +38,7 @@ class Case:
     force_axis: int = 1            # channel body-force axis (1 x, 2 y, 3 z)
     les: bool = False              # Smagorinsky LES relaxation (the reference's -DLES builds)
     cte_s: float = CTE_S_REF       # Smagorinsky constant (namelist cteS)
+    wall_axis: int = 3             # channel walls: 3 at z = 1, n (the reference), 2 at y = 1, m
     extra: dict = field(default_factory=dict)
 
     def __post_init__(self):

@@ -274,12 +274,18 @@ void dispatch_fused_les(bgk_handle& h)
                 launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_PERIODIC, false, L, SR>(h);
             break;
         case BGK_BC_CHANNEL:
-            if (c.force_axis == 2)
+            if (c.wall_axis == 2) {  // walls at y = 1, m; x and z periodic
+                if (c.force_axis == 3)
+                    launch_fused<S, AX_PERIODIC, AX_WALL, AX_PERIODIC, false, 3 | L, SR>(h);
+                else
+                    launch_fused<S, AX_PERIODIC, AX_WALL, AX_PERIODIC, false, 1 | L, SR>(h);
+            } else if (c.force_axis == 2) {
                 launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 2 | L, SR>(h);
-            else if (c.force_axis == 3)
+            } else if (c.force_axis == 3) {
                 launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 3 | L, SR>(h);
-            else
+            } else {
                 launch_fused<S, AX_PERIODIC, AX_PERIODIC, AX_WALL, false, 1 | L, SR>(h);
+            }
             break;
         default:
             if (h.has_obs)
@@ -363,6 +369,9 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
         return fail(BGK_EINVAL, "obstacles are supported with inflow or periodic boundaries");
     if (!(c.svisc > 0.0)) return fail(BGK_EINVAL, "svisc must be positive");
     if (c.force_axis < 1 || c.force_axis > 3) return fail(BGK_EINVAL, "force_axis %d", c.force_axis);
+    if (c.wall_axis != 2 && c.wall_axis != 3) return fail(BGK_EINVAL, "wall_axis %d (2: y, 3: z)", c.wall_axis);
+    if (c.boundary == BGK_BC_CHANNEL && c.wall_axis == 2 && c.force_axis == 2)
+        return fail(BGK_EINVAL, "walls at y = 1, m: the body force must lie along them (force_axis 1 or 3)");
     if (c.les != 0 && c.les != 1) return fail(BGK_EINVAL, "les %d", c.les);
     h.cfg = c;
     h.esize = storage_bytes(c.storage);
@@ -416,6 +425,7 @@ void bgk_config_default(bgk_config* c, int l, int m, int n)
     c->force_axis = 1;
     c->les = 0;
     c->cte_s = -1.0;
+    c->wall_axis = 3;
 }
 
 const char* bgk_last_error(void) { return last_error(); }

@@ -25,7 +25,7 @@ class BGKConfig(ctypes.Structure):
                 ("obs_ic", ctypes.c_int), ("obs_jc", ctypes.c_int), ("obs_kc", ctypes.c_int),
                 ("obs_radius", ctypes.c_int), ("u_inflow", ctypes.c_double), ("force_axis", ctypes.c_int),
                 ("device", ctypes.c_int), ("seed", ctypes.c_uint), ("les", ctypes.c_int),
-                ("cte_s", ctypes.c_double)]
+                ("cte_s", ctypes.c_double), ("wall_axis", ctypes.c_int)]
 
 
 def lib() -> ctypes.CDLL:
@@ -84,6 +84,7 @@ def make_config(case: Case, storage: str = "f32", rounding: int = 0, obstacle: i
     c.seed = seed
     c.les = 1 if (case.les if les is None else les) else 0
     c.cte_s = case.cte_s if cte_s is None else cte_s
+    c.wall_axis = case.wall_axis
     return c
 
 

@@ -58,6 +58,8 @@ typedef struct bgk_config {
                              SRC/col_MC.F90:381-466) */
     double cte_s;         /* Smagorinsky constant (namelist cteS); < 0 = reference default
                              (default-real 0.1, SRC/input.F90:59) */
+    int wall_axis;        /* channel walls: 3 at z = 1, n (the reference, SRC/bcond.channel.F90:
+                             255-267), 2 at y = 1, m (z then periodic; force_axis 1 or 3) */
 } bgk_config;
 
 typedef struct bgk_handle bgk_handle;

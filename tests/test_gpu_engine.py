@@ -151,7 +151,39 @@ def test_low_precision_les_close_to_f64(gpu, storage, rounding, bound):
     assert rel < bound
 
 
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-13), ("f32", 5e-6)])
+def test_channel_walls_in_y_equal_walls_in_z_swapped(gpu, storage, tol):
+    """The channel with walls at y = 1, m (wall_axis 2) is the reference's
+    channel (walls at z = 1, n) with y and z exchanged: on a square cross
+    section the fields agree after the swap, u_y and u_z exchanged, to the
+    rounding of the differently ordered sums (the oracle has only the
+    reference orientation; this pins the other one)."""
+    base = dict(svisc=0.1, u0=0.02048)
+    cz = Case("channel", 24, 16, 16, **base)
+    cy = Case("channel", 24, 16, 16, wall_axis=2, **base)
+    steps = 400
+    banner(f"channel walls y vs z {storage}")
+    with Engine(cz, storage) as e:
+        e.step(steps)
+        rz, uxz, uyz, uzz = e.fields()
+    with Engine(cy, storage) as e:
+        e.step(steps)
+        ry, uxy, uyy, uzy = e.fields()
+    T = lambda a: np.swapaxes(a, 0, 1)  # (k, j, i) -> (j, k, i)
+    scale = np.abs(uxz).max()
+    assert scale > 1e-5  # the body force has moved the fluid
+    for a, b in ((T(uxy), uxz), (T(uyy), uzz), (T(uzy), uyz)):
+        err = np.abs(a - b).max() / scale
+        print(f"[gpu] walls y vs z {storage}: max rel diff {err:.2e}", flush=True)
+        assert err < tol
+    np.testing.assert_allclose(T(ry), rz, rtol=tol, atol=0)
+
+
 def test_engine_rejects_bad_config(gpu):
     from bgk3d_gpu_amd import runtime as rt
     with pytest.raises(rt.BGKError):
         Engine(Case("ldc", 8, 8, 8), "f32", obstacle=1)  # obstacles need inflow/periodic
+    with pytest.raises(rt.BGKError):  # walls at y: the force must lie along them
+        Engine(Case("channel", 8, 8, 8, wall_axis=2, force_axis=2), "f32")
+    with pytest.raises(rt.BGKError):
+        Engine(Case("channel", 8, 8, 8, wall_axis=1), "f32")
