@@ -393,6 +393,54 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FusedWaves<
     fused_rows<S, BX, BY, BZ, OBST, FMODE, SR>(p, rm, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, p.seed);
 }
 
+// Reference-layout form (the caller's arrays, every boundary already in the
+// halo; FP32/FP64).  The waves of plane k cover its interior rows j = 1..m as
+// ONE range of cells, from (1, 1, k) to (l, m, k): 64 consecutive cells per
+// wave, the first wave starting on the 128-byte line of the output that
+// holds (1, 1, k).  A wave may span the end of one row and the start of the
+// next; its lanes on halo or padding cells (i = 0, i > l) neither load nor
+// store.  Row-by-row waves need ceil((l + shift) / 64) waves per row when
+// rows do not start on a line (nx = l + 2: five waves, the fifth holding
+// 0-31 cells, for l = 256); this form needs (nx*(m-1) + l + shift) / 64 per
+// plane.  blockIdx.y = k - 1; the plane's cells must be indexable in 32 bits
+// (nx * ny < 2^31, checked by the launcher).
+template <typename S, int FMODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FusedWaves<S, FMODE>::value))) void collide_flat(
+    FusedArgs<S> p)
+{
+    static_assert(sizeof(S) >= 4, "the reference layout holds FP32 or FP64 populations");
+    typedef typename Storage<S>::compute C;
+    constexpr bool LES = (FMODE & 4) != 0;
+    const int k = 1 + (int)blockIdx.y;
+    const long plane = p.nxy * (long)k;
+    const int lo = p.nx + 1;             // (1, 1) within the plane
+    const int hi = p.nx * p.m + p.l;     // (l, m)
+    const long b0 = (long)(reinterpret_cast<uintptr_t>(p.b[0]) / sizeof(S));
+    const int start = lo - (int)((b0 + plane + lo) & (long)(p.align - 1));
+    const int r = start + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r < lo || r > hi) return;
+    const int j = (int)((unsigned)r / (unsigned)p.nx);
+    const int i = r - j * p.nx;
+    if (i < 1 || i > p.l) return;
+    const long row = plane + (long)(r - i);
+
+    C x[19];
+    pull_all<S, AX_HALO, AX_HALO, AX_HALO, false, false, false>(p, i, j, k, row, x);
+    C fx, fy, fz;
+    force_terms<C, (FMODE & 3)>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
+    if constexpr (sizeof(S) == 8 && !LES) {
+        bgk_collide_stream(x, p.omega, fx, fy, fz, p.cst, [&](auto dc, C v) {
+            constexpr int d = decltype(dc)::value;
+            store_nt((d < 18 ? p.b[d] : p.a[18]) + (row + i), v);
+        });
+    } else {
+        C o[19];
+        bgk_collide_ref<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
+#pragma unroll
+        for (int d = 0; d < 19; ++d) store_nt((d < 18 ? p.b[d] : p.a[18]) + (row + i), o[d]);
+    }
+}
+
 // Launch geometry shared by both ABIs; `span` is the row length the waves
 // must cover (l, or l plus the largest alignment shift).
 inline dim3 fused_grid(const RowMap& rm, int m, int n)

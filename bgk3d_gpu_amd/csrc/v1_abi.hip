@@ -128,13 +128,27 @@ void launch_col(const Lattice19<real_t>& A, const Lattice19<real_t>& B, const Ge
     p.align = (int)(128 / sizeof(real_t));
     p.lend = 0;  // the caller's halo and padding are left untouched
     p.seed = 0;
-    // when every row shares one alignment the exact shift sizes the grid
+    // Row by row: ceil(span / 64) waves per row, span = l plus the rows'
+    // alignment shift (exact when every row shares one alignment, else the
+    // largest).  Flat: the plane's interior rows as one line-aligned range,
+    // (nx*(m-1) + l + shift) / 64 waves per plane.  The mapping with fewer
+    // waves is launched: flat when a row's last wave would be mostly idle
+    // (l = 256, nx = 258: 4.03 waves per row instead of 5), row by row when
+    // rows fill their waves (l = 62 from a line start) or carry wide padding.
     int span = g.l + p.align - 1;
     if (g.nx % p.align == 0 && g.nxy % p.align == 0) {
         const long first = (long)(reinterpret_cast<uintptr_t>(p.b[0]) / sizeof(real_t)) + 1;
         span = g.l + (int)(first & (p.align - 1));
     }
     const RowMap rm = row_map(span);
+    const long row_waves = (long)g.m * rm.wpr;
+    const long flat_cells = (long)g.nx * (g.m - 1) + g.l + p.align - 1;
+    if (g.nxy < (1L << 31) && ceil_div(flat_cells, 64L) < row_waves) {
+        const dim3 grid((unsigned)ceil_div(flat_cells, 256L), (unsigned)g.n);
+        hipLaunchKernelGGL((collide_flat<real_t, FM>), grid, dim3(256), 0, abi_stream(), p);
+        report_launch("collide_flat");
+        return;
+    }
     hipLaunchKernelGGL((collide_fused<real_t, AX_HALO, AX_HALO, AX_HALO, false, FM, 0>), fused_grid(rm, g.m, g.n),
                        fused_block(rm), 0, abi_stream(), p, rm);
     report_launch("collide_fused");

@@ -3,7 +3,9 @@ U0 = 0.05, handle API.
 
 * Fields: after 3 steps from the reference's initial state (its shifted
   storage drops the pressure field) the FP32 and FP64 populations are
-  bit-identical to the oracle's at the full 256^3 size.
+  bit-identical to the oracle's at the full 256^3 size, through the handle
+  API and through the reference-compatible ABI (caller-owned arrays with
+  the halo filled by bcond_periodic_gpu, nx = l + 2 or padded rows).
 * Energy: from the state with its pressure field, the kinetic energy after
   1000 steps follows the viscous decay exp(-6 nu k0^2 t), k0 = 2 pi / 256,
   to the stated tolerance (the flow is weakly nonlinear, Re = U0/(nu k0) = 204).
@@ -18,6 +20,7 @@ import pytest
 import oracle_support as O
 from bgk3d_gpu_amd.cases import Case
 from bgk3d_gpu_amd.engine import Engine
+from bgk3d_gpu_amd.simulation import Simulation
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +50,27 @@ def test_full_size_fields_match_oracle(gpu, storage):
           nthreads=min(16, os.cpu_count() or 1))
     oi = A[:, 1:-1, 1:-1, 1:-1].astype(np.float64)
     assert np.array_equal(G, oi), f"max |diff| {np.abs(G - oi).max()}"
+
+
+# ipad 0 (rows of 258 cells): waves over each plane's interior rows as one
+# range, 4.03 waves per row; ipad 70 (rows of 328 cells): row-by-row waves,
+# 5 per row, fewer than the 5.1 the padded range would take.
+@pytest.mark.parametrize("prec,ipad", [("sp", 0), ("sp", 70), ("dp", 0)])
+def test_full_size_reference_abi_matches_handle(gpu, prec, ipad):
+    storage = {"sp": "f32", "dp": "f64"}[prec]
+    steps = 3
+    banner(f"Taylor-Green {L}^3 {prec} ipad={ipad}: reference ABI against the handle, {steps} steps")
+    with Engine(Case("periodic", L, L, L, svisc=NU, u0=U0, init=1), storage) as eng:
+        eng.step(steps)
+        G = eng.populations()
+    sim = Simulation(Case("periodic", L, L, L, svisc=NU, u0=U0, init=1, ipad=ipad), prec)
+    try:
+        sim.step(steps)
+        A = sim.state()
+    finally:
+        sim.close()
+    V = A[:, 1:L + 1, 1:L + 1, 1:L + 1].astype(np.float64)
+    assert np.array_equal(V, G), f"max |diff| {np.abs(V - G).max()}"
 
 
 def test_kinetic_energy_follows_viscous_decay(gpu):

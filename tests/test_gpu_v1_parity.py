@@ -26,6 +26,12 @@ CASES = {
     "periodic": Case("periodic", 20, 16, 12, svisc=0.01, u0=0.05, init=1),
     "channel": Case("channel", 24, 16, 12, svisc=0.1, u0=0.02048),
     "inflow": Case("inflow", 40, 24, 24, svisc=0.02, u0=0.1, obstacle=True),
+    # the cases above run the collide with waves over each plane's interior
+    # rows as one range; these two keep the row-by-row mapping (fewer
+    # waves): rows of 64 cells on 128-byte lines, and rows padded past
+    # 2.5 waves of cells
+    "periodic_rows": Case("periodic", 62, 10, 6, svisc=0.01, u0=0.05, init=1),
+    "ldc_wide_pad": Case("ldc", 40, 6, 5, svisc=0.02, u0=0.1, ipad=100),
 }
 
 
