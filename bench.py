@@ -108,23 +108,11 @@ def load_traffic(workload):
 
 def bench_v1(case, prec, steps, warmup):
     """Reference-compatible ABI driven the reference's way (one
-    bcond_periodic_gpu + col_mc_gpu per step, caller-owned arrays)."""
-    from bgk3d_gpu_amd import runtime as rt
-    from bgk3d_gpu_amd.simulation import Simulation
+    bcond_periodic_gpu + col_mc_gpu per step, caller-owned arrays); total
+    milliseconds of the timed steps."""
+    from bgk3d_gpu_amd.studies.reference_abi import time_steps
 
-    sim = Simulation(case, prec, poison=False)
-    lib = sim.lib
-    try:
-        sim.step(warmup)
-        sim.sync()
-        ev_start, ev_stop = rt.Event(lib), rt.Event(lib)
-        ev_start.record()
-        sim.step(steps)
-        ev_stop.record()
-        total_ms = ev_start.elapsed_ms(ev_stop)
-    finally:
-        sim.close()
-    return total_ms
+    return time_steps(case, prec, steps, warmup)
 
 
 def bench_engine(case, storage, steps, warmup, want_ke=False, rounding=None):

@@ -1,0 +1,57 @@
+"""Rate of the reference-compatible ABI on periodic Taylor-Green, by row padding.
+
+Drives libbgk3d_gpu_{sp,dp}.so the reference's way (bcond_periodic_gpu, then
+col_mc_gpu, then the swap, every step; caller-owned arrays of
+(0:l+1+ipad, 0:m+1, 0:n+1)) and prints one JSON line per precision and
+padding: the mean step time from hipEvents around the timed steps, and
+MLUPS.  With ipad = 0 the rows hold 258 values and start anywhere in a
+128-byte line; larger paddings show how the launcher's choice between
+row-by-row waves and waves over each plane's interior rows plays out.
+
+    python -m bgk3d_gpu_amd.studies.reference_abi > reference_abi.json
+    python -m bgk3d_gpu_amd.studies.reference_abi --prec sp --ipad 0 --steps 20
+"""
+from __future__ import annotations
+
+import argparse
+import json
+
+from .. import runtime as rt
+from ..cases import Case
+from ..simulation import Simulation
+
+
+def time_steps(case: Case, prec: str, steps: int, warmup: int) -> float:
+    sim = Simulation(case, prec, poison=False)
+    try:
+        sim.step(warmup)
+        sim.sync()
+        start, stop = rt.Event(sim.lib), rt.Event(sim.lib)
+        start.record()
+        sim.step(steps)
+        stop.record()
+        return start.elapsed_ms(stop)
+    finally:
+        sim.close()
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--prec", default="sp,dp", help="comma-separated: sp, dp")
+    ap.add_argument("--ipad", default="0,1,30,70", help="comma-separated row paddings")
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    args = ap.parse_args(argv)
+    L = args.size
+    for prec in args.prec.split(","):
+        for ipad in (int(x) for x in args.ipad.split(",")):
+            case = Case("periodic", L, L, L, svisc=0.01, u0=0.05, init=1, ipad=ipad)
+            ms = time_steps(case, prec, args.steps, args.warmup)
+            print(json.dumps({"grid": [L, L, L], "prec": prec, "ipad": ipad, "nx": L + 2 + ipad,
+                              "step_us": round(ms / args.steps * 1e3, 1),
+                              "mlups": round(case.cells * args.steps / (ms * 1e-3) / 1e6, 1)}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
