@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "host_init.hpp"
+#include "init_kernels.hpp"
 #include "io_kernels.hpp"
 
 using namespace bgk;
@@ -59,18 +60,6 @@ namespace {
 // ---------------------------------------------------------------------------
 // kernels specific to the engine
 // ---------------------------------------------------------------------------
-
-// `src` holds `n` values: consecutive z planes of `plane` values each; plane
-// q of them goes to dst + q * zstride (the lattice's z stride).
-template <typename S>
-__global__ void convert_in(const typename Storage<S>::compute* __restrict__ src, S* __restrict__ dst, long n,
-                           long plane, long zstride)
-{
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const long q = t / plane;
-    store_value(dst + (q * zstride + (t - q * plane)), Rounder<S, 0>::cvt(src[t], 0u));
-}
 
 // Inflow halo x = 0, directions 1..5, every (j,k) incl. halo rows: the
 // shifted equilibrium for rho = 1, u = (u_inflow, 0, 0) of
@@ -204,38 +193,15 @@ int upload_state(bgk_handle& h)
     const bgk_config& c = h.cfg;
     // rest state: shifted populations are all zero, which the memset left
     if (c.init == 1 || c.init == 2) {
-        // Taylor-Green: host f_eq slab by slab (<= 16 planes of all 19 populations)
-        const int slab = 16;
-        const long plane = h.plane;
-        std::vector<std::vector<C>> host(19, std::vector<C>((size_t)(plane * slab), C(0)));
-        C* ptrs[19];
-        for (int d = 0; d < 19; ++d) ptrs[d] = host[d].data();
-        C* tmp = nullptr;
-        ENG_HIP(hipMalloc(&tmp, plane * slab * sizeof(C)));
+        // Taylor-Green: sin/cos factors per coordinate on the host, f_eq per
+        // cell on the device (bit-identical to the host form), both lattices
         const C u00 = (C)(c.boundary == BGK_BC_CHANNEL ? 0.0 : c.u0);
-        int rc = 0;
-        for (int k0 = 0; k0 <= c.n + 1 && rc == 0; k0 += slab) {
-            const int k1 = std::min(c.n + 1, k0 + slab - 1);
-            const long cnt = plane * (k1 - k0 + 1);
-            if (init_feq_host<C>(ptrs, c.l, c.m, c.n, h.nx, h.ny, c.init, u00, k0, k1) != 0) {
-                rc = fail(BGK_EINVAL, "init mode %d not supported", c.init);
-                break;
-            }
-            for (int d = 0; d < 19 && rc == 0; ++d) {
-                if (hipMemcpy(tmp, ptrs[d], cnt * sizeof(C), hipMemcpyHostToDevice) != hipSuccess) {
-                    rc = fail(BGK_EHIP, "upload of population %d failed", d);
-                    break;
-                }
-                for (int buf = 0; buf < 2; ++buf) {
-                    if (buf == 1 && d == 18) continue;
-                    hipLaunchKernelGGL(convert_in<S>, dim3(ceil_div(cnt, 256)), dim3(256), 0, h.stream, tmp,
-                                       h.arr<S>(buf, d) + h.nxy * k0, cnt, plane, h.nxy);
-                }
-                if (hipStreamSynchronize(h.stream) != hipSuccess) rc = fail(BGK_EHIP, "conversion failed");
-            }
+        for (int buf = 0; buf < 2; ++buf) {
+            S* ptrs[19];
+            for (int d = 0; d < 19; ++d) ptrs[d] = (buf == 1 && d == 18) ? nullptr : h.arr<S>(buf, d);
+            const hipError_t e = init_feq_device<S>(ptrs, c.l, c.m, c.n, h.nx, h.nxy, c.init, u00, h.stream);
+            if (e != hipSuccess) return fail(BGK_EHIP, "device init failed: %s", hipGetErrorString(e));
         }
-        hipFree(tmp);
-        if (rc) return rc;
     } else if (c.init != 0) {
         return fail(BGK_EINVAL, "init mode %d not supported", c.init);
     }

@@ -17,6 +17,7 @@
 #include "reduce.hpp"
 #include "v1_kernels.hpp"
 #include "host_init.hpp"
+#include "init_kernels.hpp"
 
 #include "../../include/bgk3d_gpu.h"
 
@@ -462,6 +463,21 @@ int probe_gpu(real_t** a, int l, int m, int n, int nx, int ny, int i0, int j0, i
 int bgk_init_host(real_t** h, int l, int m, int n, int nx, int ny, int mode, real_t u00)
 {
     return bgk::init_feq_host<real_t>(h, l, m, n, nx, ny, mode, u00);
+}
+
+int bgk_init_gpu(real_t** a, int l, int m, int n, int nx, int ny, int mode, real_t u00)
+{
+    if (!shape_ok("bgk_init_gpu", l, m, n, nx, ny, a)) return -1;
+    if (mode < 0 || mode > 2) {
+        std::fprintf(stderr, "bgk3d_gpu: bgk_init_gpu: init mode %d not supported (0, 1, 2)\n", mode);
+        return -1;
+    }
+    const hipError_t e = bgk::init_feq_device<real_t>(a, l, m, n, nx, (long)nx * ny, mode, u00, abi_stream());
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "bgk3d_gpu: bgk_init_gpu: %s\n", hipGetErrorString(e));
+        return -2;
+    }
+    return 0;
 }
 
 // ---- device memory plumbing for hosts that have no other HIP runtime ----

@@ -174,6 +174,14 @@
           real(c_real),   value :: u00
         end function bgk_init_host
 
+        integer(c_int) function bgk_init_gpu(a,l,m,n,nx,ny,mode,u00)     &
+                   bind(C, name="bgk_init_gpu")
+          import :: c_ptr, c_int, c_real
+          type(c_ptr)           :: a(19)
+          integer(c_int), value :: l,m,n,nx,ny,mode
+          real(c_real),   value :: u00
+        end function bgk_init_gpu
+
         integer(c_int) function bgk_dev_malloc(ptr,bytes)                &
                    bind(C, name="bgk_dev_malloc")
           import :: c_ptr, c_size_t

@@ -66,6 +66,15 @@ class Simulation:
                  "bgk_init_host")
         return host
 
+    def init_device(self, mode=None, u00=None):
+        """Write the initial state straight into the device arrays with
+        bgk_init_gpu (bit-identical to initial_state(); padding untouched)."""
+        c = self.case
+        mode = c.init if mode is None else mode
+        u00 = c.u00 if u00 is None else u00
+        self.lib.bgk_init_gpu.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [self.Rc]
+        rt.check(self.lib.bgk_init_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, mode, self.Rc(u00)), "bgk_init_gpu")
+
     def set_state(self, host: np.ndarray):
         host = np.ascontiguousarray(host, dtype=self.R)
         assert host.shape == (19,) + self.case.shape

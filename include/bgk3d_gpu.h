@@ -164,6 +164,14 @@ int probe_gpu(bgk_real_t** a, int l, int m, int n, int nx, int ny, int i0, int j
  * Replaces SRC/init.F90:31-209 (PERIODIC + TGV3D branch for mode 1). */
 int bgk_init_host(bgk_real_t** h, int l, int m, int n, int nx, int ny, int mode, bgk_real_t u00);
 
+/* The same state written directly into the DEVICE arrays a[0..18] (all
+ * cells incl. halo; padding untouched), bit-identical to bgk_init_host: the
+ * host evaluates sin/cos once per coordinate, a kernel forms every cell.
+ * Waits for completion; 0 on success, -1 bad argument, -2 HIP error.
+ * Replaces the host init plus the full-lattice upload the reference does
+ * (SRC/init.F90:31-209, then the OpenMP target data map, SRC/bgk3d.F90:163). */
+int bgk_init_gpu(bgk_real_t** a, int l, int m, int n, int nx, int ny, int mode, bgk_real_t u00);
+
 /* Select the in-order stream (a hipStream_t) all entry points queue on;
  * NULL restores the null stream.  No counterpart in the reference. */
 int bgk_set_stream(void* stream);
