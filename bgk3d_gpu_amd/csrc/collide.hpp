@@ -149,7 +149,12 @@ __device__ __forceinline__ typename Storage<S>::compute pull_load(const S* ptr)
 // may bounce off an x wall or a solid cell, where each lane picks its own
 // address.  The 32/64-bit forms are bound by HBM and keep per-lane 64-bit
 // addresses, which leaves the scalar registers free for higher occupancy.
-template <typename S, int BX, int BY, int BZ, bool OBST, bool EDGE, bool CMP>
+// NT0: the c_x = 0 directions load non-temporally (the default; right when
+// the y and z shifts keep every wave on whole lines, as in the engine's
+// layout).  Off when rows or planes do not start on a line: a y or z shifted
+// line is then shared by two neighbouring waves, and a streaming load would
+// fetch it twice.
+template <typename S, int BX, int BY, int BZ, bool OBST, bool EDGE, bool CMP, bool NT0 = true>
 __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, int k, long row,
                                          typename Storage<S>::compute (&x)[19])
 {
@@ -192,7 +197,7 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
             const long s = row + i;
             const long src = s + ds + dx;
             const bool b = wall || bounce;
-            v = pull_load<S, cx>(b ? (p.a[opp_of(d)] + s) : (p.a[d] + src));
+            v = pull_load<S, NT0 ? cx : 1>(b ? (p.a[opp_of(d)] + s) : (p.a[d] + src));
             if (BZ == AX_LID && EDGE) {
                 if (d == 1 && top) v = v + p.lidforce;
                 if (d == 10 && top) v = v - p.lidforce;
@@ -202,16 +207,16 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
         }
         const char* own = reinterpret_cast<const char*>(pop_a<CMP>(p, opp_of(d))) + rowb;
         if (wall) {  // y or z wall: the whole wave bounces
-            v = pull_load<S, cx>(reinterpret_cast<const S*>(own + (uint64_t)vi));
+            v = pull_load<S, NT0 ? cx : 1>(reinterpret_cast<const S*>(own + (uint64_t)vi));
         } else {
             const char* base = reinterpret_cast<const char*>(pop_a<CMP>(p, d)) + rowb + ds * ES;
             const S* nptr = reinterpret_cast<const S*>(base + (uint64_t)(vi + (uint32_t)dx * (uint32_t)ES));
             if ((BX == AX_WALL && cx != 0) || (OBST && d < 18)) {
                 // a lane may bounce: pick its address, one load
                 const S* optr = reinterpret_cast<const S*>(own + (uint64_t)vi);
-                v = pull_load<S, cx>(bounce ? optr : nptr);
+                v = pull_load<S, NT0 ? cx : 1>(bounce ? optr : nptr);
             } else {
-                v = pull_load<S, cx>(nptr);
+                v = pull_load<S, NT0 ? cx : 1>(nptr);
             }
         }
         if (BZ == AX_LID && EDGE) {
@@ -403,8 +408,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FusedWaves<
 // rows do not start on a line (nx = l + 2: five waves, the fifth holding
 // 0-31 cells, for l = 256); this form needs (nx*(m-1) + l + shift) / 64 per
 // plane.  blockIdx.y = k - 1; the plane's cells must be indexable in 32 bits
-// (nx * ny < 2^31, checked by the launcher).
-template <typename S, int FMODE>
+// (nx * ny < 2^31, checked by the launcher).  NT0 as for pull_all: set only
+// when nx and nx*ny are multiples of a line.
+template <typename S, int FMODE, bool NT0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FusedWaves<S, FMODE>::value))) void collide_flat(
     FusedArgs<S> p)
 {
@@ -425,7 +431,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FusedWaves<
     const long row = plane + (long)(r - i);
 
     C x[19];
-    pull_all<S, AX_HALO, AX_HALO, AX_HALO, false, false, false>(p, i, j, k, row, x);
+    pull_all<S, AX_HALO, AX_HALO, AX_HALO, false, false, false, NT0>(p, i, j, k, row, x);
     C fx, fy, fz;
     force_terms<C, (FMODE & 3)>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
     if constexpr (sizeof(S) == 8 && !LES) {

@@ -146,7 +146,10 @@ void launch_col(const Lattice19<real_t>& A, const Lattice19<real_t>& B, const Ge
     const long flat_cells = (long)g.nx * (g.m - 1) + g.l + p.align - 1;
     if (g.nxy < (1L << 31) && ceil_div(flat_cells, 64L) < row_waves) {
         const dim3 grid((unsigned)ceil_div(flat_cells, 256L), (unsigned)g.n);
-        hipLaunchKernelGGL((collide_flat<real_t, FM>), grid, dim3(256), 0, abi_stream(), p);
+        if (g.nx % p.align == 0 && g.nxy % p.align == 0)
+            hipLaunchKernelGGL((collide_flat<real_t, FM, true>), grid, dim3(256), 0, abi_stream(), p);
+        else
+            hipLaunchKernelGGL((collide_flat<real_t, FM, false>), grid, dim3(256), 0, abi_stream(), p);
         report_launch("collide_flat");
         return;
     }
