@@ -287,9 +287,7 @@ void bcond_driven_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t fo
     if (!shape_ok("bcond_driven_gpu", l, m, n, nx, ny, a)) return;
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
-    launch_face(driven_x<real_t>, (long)m * n, "driven_x", A, g);
-    launch_face(driven_y<real_t>, (long)l * n, "driven_y", A, g);
-    launch_face(walls_z<real_t>, (long)l * m, "walls_z", A, g, force, 1);
+    launch_face(driven_faces<real_t>, (long)m * n + (long)l * n + (long)l * m, "driven_faces", A, g, force);
 }
 
 void bcond_inflow_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t u_inflow, real_t rf, real_t qf,

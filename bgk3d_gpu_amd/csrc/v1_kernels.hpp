@@ -35,42 +35,46 @@ __device__ __forceinline__ long gidx(const Geometry& g, int i, int j, int k)
 // Boundary faces.  A(d) is population d (1-based, as in the Fortran).
 // -------------------------------------------------------------------------
 #define BGK_A(d, i, j, k) a.p[(d)-1][gidx(g, (i), (j), (k))]
-
-// bcond_driven x faces, j=1..m, k=1..n (SRC/bcond.driven.F90:105-119)
+// bcond_driven x faces at (j, k), j=1..m, k=1..n (SRC/bcond.driven.F90:105-119).
+// The y and z faces are written in the same launch (driven_faces), so the
+// writes they overwrite (the same population of the same halo cell) are
+// skipped here; the reference's face order x, y, z makes the later value the
+// one kept (SRC/bcond.driven.F90:105-247).
 template <typename R>
-__global__ void driven_x(Lattice19<R> a, Geometry g)
+__device__ __forceinline__ void driven_x_cell(Lattice19<R>& a, const Geometry& g, int j, int k)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.m * g.n) return;
-    const int j = 1 + t % g.m, k = 1 + t / g.m, l = g.l, l1 = g.l + 1;
-    BGK_A(12, l1, j - 1, k) = BGK_A(1, l, j, k);
-    BGK_A(13, l1, j, k - 1) = BGK_A(2, l, j, k);
-    BGK_A(10, l1, j + 1, k) = BGK_A(3, l, j, k);
-    BGK_A(11, l1, j, k + 1) = BGK_A(4, l, j, k);
+    const int l = g.l, l1 = g.l + 1;
+    const bool y_lo = j == 1, y_hi = j == g.m;  // the y faces rewrite these
+    const bool z_lo = k == 1, z_hi = k == g.n;  // the z faces rewrite these
+    if (!y_lo) BGK_A(12, l1, j - 1, k) = BGK_A(1, l, j, k);
+    if (!z_lo) BGK_A(13, l1, j, k - 1) = BGK_A(2, l, j, k);
+    if (!y_hi) BGK_A(10, l1, j + 1, k) = BGK_A(3, l, j, k);
+    if (!z_hi) BGK_A(11, l1, j, k + 1) = BGK_A(4, l, j, k);
     BGK_A(14, l1, j, k) = BGK_A(5, l, j, k);
-    BGK_A(3, 0, j - 1, k) = BGK_A(10, 1, j, k);
-    BGK_A(4, 0, j, k - 1) = BGK_A(11, 1, j, k);
-    BGK_A(1, 0, j + 1, k) = BGK_A(12, 1, j, k);
-    BGK_A(2, 0, j, k + 1) = BGK_A(13, 1, j, k);
+    if (!y_lo) BGK_A(3, 0, j - 1, k) = BGK_A(10, 1, j, k);
+    if (!z_lo) BGK_A(4, 0, j, k - 1) = BGK_A(11, 1, j, k);
+    if (!y_hi) BGK_A(1, 0, j + 1, k) = BGK_A(12, 1, j, k);
+    if (!z_hi) BGK_A(2, 0, j, k + 1) = BGK_A(13, 1, j, k);
     BGK_A(5, 0, j, k) = BGK_A(14, 1, j, k);
 }
 
-// bcond_driven y faces, i=1..l, k=1..n (SRC/bcond.driven.F90:169-183)
+// bcond_driven y faces at (i, k), i=1..l, k=1..n (SRC/bcond.driven.F90:169-183);
+// the z faces follow in the same launch, so their cells are skipped as in
+// driven_x_cell.
 template <typename R>
-__global__ void driven_y(Lattice19<R> a, Geometry g)
+__device__ __forceinline__ void driven_y_cell(Lattice19<R>& a, const Geometry& g, int i, int k)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.l * g.n) return;
-    const int i = 1 + t % g.l, k = 1 + t / g.l, m = g.m, m1 = g.m + 1;
+    const int m = g.m, m1 = g.m + 1;
+    const bool z_lo = k == 1, z_hi = k == g.n;
     BGK_A(8, i, 0, k) = BGK_A(17, i, 1, k);
     BGK_A(12, i + 1, 0, k) = BGK_A(1, i, 1, k);
     BGK_A(3, i - 1, 0, k) = BGK_A(10, i, 1, k);
-    BGK_A(7, i, 0, k - 1) = BGK_A(16, i, 1, k);
-    BGK_A(9, i, 0, k + 1) = BGK_A(18, i, 1, k);
+    if (!z_lo) BGK_A(7, i, 0, k - 1) = BGK_A(16, i, 1, k);
+    if (!z_hi) BGK_A(9, i, 0, k + 1) = BGK_A(18, i, 1, k);
     BGK_A(10, i + 1, m1, k) = BGK_A(3, i, m, k);
-    BGK_A(16, i, m1, k + 1) = BGK_A(7, i, m, k);
+    if (!z_hi) BGK_A(16, i, m1, k + 1) = BGK_A(7, i, m, k);
     BGK_A(17, i, m1, k) = BGK_A(8, i, m, k);
-    BGK_A(18, i, m1, k - 1) = BGK_A(9, i, m, k);
+    if (!z_lo) BGK_A(18, i, m1, k - 1) = BGK_A(9, i, m, k);
     BGK_A(1, i - 1, m1, k) = BGK_A(12, i, m, k);
 }
 
@@ -78,11 +82,9 @@ __global__ void driven_y(Lattice19<R> a, Geometry g)
 // `lid` = 1 moves the top wall (driven cavity), 0 makes it a plain no-slip
 // wall (the channel walls of SRC/bcond.channel.F90:244-267).
 template <typename R>
-__global__ void walls_z(Lattice19<R> a, Geometry g, R force, int lid)
+__device__ __forceinline__ void walls_z_cell(Lattice19<R>& a, const Geometry& g, int i, int j, R force, int lid)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.l * g.m) return;
-    const int i = 1 + t % g.l, j = 1 + t / g.l, n = g.n, n1 = g.n + 1;
+    const int n = g.n, n1 = g.n + 1;
     BGK_A(6, i, j, 0) = BGK_A(15, i, j, 1);
     BGK_A(4, i - 1, j, 0) = BGK_A(11, i, j, 1);
     BGK_A(7, i, j - 1, 0) = BGK_A(16, i, j, 1);
@@ -101,6 +103,36 @@ __global__ void walls_z(Lattice19<R> a, Geometry g, R force, int lid)
         BGK_A(11, i + 1, j, n1) = BGK_A(4, i, j, n);
     }
     BGK_A(16, i, j + 1, n1) = BGK_A(7, i, j, n);
+}
+
+template <typename R>
+__global__ void walls_z(Lattice19<R> a, Geometry g, R force, int lid)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.l * g.m) return;
+    walls_z_cell(a, g, 1 + t % g.l, 1 + t / g.l, force, lid);
+}
+
+// bcond_driven in one launch: threads [0, m*n) take the x faces, the next
+// l*n the y faces, the last l*m the z faces with the lid.  Every value read
+// is an interior cell, which no face writes, so the faces interact only
+// where two of them write the same population of the same halo cell; there
+// the earlier face (in the reference's order x, y, z) skips its write and the
+// result equals the three sequential launches.
+template <typename R>
+__global__ void driven_faces(Lattice19<R> a, Geometry g, R force)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nx_f = (long)g.m * g.n, ny_f = (long)g.l * g.n, nz_f = (long)g.l * g.m;
+    if (t < nx_f) {
+        driven_x_cell<R>(a, g, 1 + (int)(t % g.m), 1 + (int)(t / g.m));
+    } else if (t < nx_f + ny_f) {
+        const long u = t - nx_f;
+        driven_y_cell<R>(a, g, 1 + (int)(u % g.l), 1 + (int)(u / g.l));
+    } else if (t < nx_f + ny_f + nz_f) {
+        const long u = t - nx_f - ny_f;
+        walls_z_cell(a, g, 1 + (int)(u % g.l), 1 + (int)(u / g.l), force, 1);
+    }
 }
 
 // periodic x face over j=0..m+1, k=0..n+1 (SRC/bcond.periodic.F90:105-119)

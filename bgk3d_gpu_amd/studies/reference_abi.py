@@ -1,15 +1,17 @@
-"""Rate of the reference-compatible ABI on periodic Taylor-Green, by row padding.
+"""Rate of the reference-compatible ABI by row padding: periodic Taylor-Green,
+or the lid-driven cavity (--boundary ldc).
 
-Drives libbgk3d_gpu_{sp,dp}.so the reference's way (bcond_periodic_gpu, then
-col_mc_gpu, then the swap, every step; caller-owned arrays of
+Drives libbgk3d_gpu_{sp,dp}.so the reference's way (bcond_periodic_gpu or
+bcond_driven_gpu, then col_mc_gpu, then the swap, every step; caller-owned arrays of
 (0:l+1+ipad, 0:m+1, 0:n+1)) and prints one JSON line per precision and
 padding: the mean step time from hipEvents around the timed steps, and
-MLUPS.  With ipad = 0 the rows hold 258 values and start anywhere in a
-128-byte line; larger paddings show how the launcher's choice between
+MLUPS.  With ipad = 0 the rows hold l + 2 values (258 at the default size)
+and start anywhere in a 128-byte line; larger paddings show how the launcher's choice between
 row-by-row waves and waves over each plane's interior rows plays out.
 
     python -m bgk3d_gpu_amd.studies.reference_abi > reference_abi.json
     python -m bgk3d_gpu_amd.studies.reference_abi --prec sp --ipad 0 --steps 20
+    python -m bgk3d_gpu_amd.studies.reference_abi --boundary ldc --size 64 --ipad 0
 """
 from __future__ import annotations
 
@@ -38,6 +40,7 @@ def time_steps(case: Case, prec: str, steps: int, warmup: int) -> float:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--boundary", choices=["periodic", "ldc"], default="periodic")
     ap.add_argument("--prec", default="sp,dp", help="comma-separated: sp, dp")
     ap.add_argument("--ipad", default="0,1,30,70", help="comma-separated row paddings")
     ap.add_argument("--steps", type=int, default=40)
@@ -46,10 +49,13 @@ def main(argv=None):
     L = args.size
     for prec in args.prec.split(","):
         for ipad in (int(x) for x in args.ipad.split(",")):
-            case = Case("periodic", L, L, L, svisc=0.01, u0=0.05, init=1, ipad=ipad)
+            if args.boundary == "periodic":
+                case = Case("periodic", L, L, L, svisc=0.01, u0=0.05, init=1, ipad=ipad)
+            else:  # Re = 1000 with the reference's lid speed
+                case = Case("ldc", L, L, L, svisc=0.1 * L / 1000.0, u0=0.1, ipad=ipad)
             ms = time_steps(case, prec, args.steps, args.warmup)
-            print(json.dumps({"grid": [L, L, L], "prec": prec, "ipad": ipad, "nx": L + 2 + ipad,
-                              "step_us": round(ms / args.steps * 1e3, 1),
+            print(json.dumps({"boundary": args.boundary, "grid": [L, L, L], "prec": prec, "ipad": ipad,
+                              "nx": L + 2 + ipad, "step_us": round(ms / args.steps * 1e3, 1),
                               "mlups": round(case.cells * args.steps / (ms * 1e-3) / 1e6, 1)}), flush=True)
 
 
