@@ -267,7 +267,7 @@ void bcond_periodic_gpu(real_t** a, int l, int m, int n, int nx, int ny)
     if (!shape_ok("bcond_periodic_gpu", l, m, n, nx, ny, a)) return;
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
-    launch_face(periodic_x<real_t>, (long)(m + 2) * (n + 2), "periodic_x", A, g);
+    launch_face(periodic_x<real_t>, 10 * x_face_rows(g), "periodic_x", A, g);
     launch_face(periodic_y<real_t>, (long)(l + 2) * (n + 2), "periodic_y", A, g);
     launch_face(periodic_z<real_t>, (long)(l + 2) * (m + 2), "periodic_z", A, g);
 }
@@ -277,7 +277,7 @@ void bcond_channel_gpu(real_t** a, int l, int m, int n, int nx, int ny)
     if (!shape_ok("bcond_channel_gpu", l, m, n, nx, ny, a)) return;
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
-    launch_face(periodic_x<real_t>, (long)(m + 2) * (n + 2), "periodic_x", A, g);
+    launch_face(periodic_x<real_t>, 10 * x_face_rows(g), "periodic_x", A, g);
     launch_face(periodic_y<real_t>, (long)(l + 2) * (n + 2), "periodic_y", A, g);
     launch_face(walls_z<real_t>, (long)l * m, "walls_z", A, g, real_t(0), 0);
 }

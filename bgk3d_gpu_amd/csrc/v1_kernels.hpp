@@ -135,18 +135,29 @@ __global__ void driven_faces(Lattice19<R> a, Geometry g, R force)
     }
 }
 
-// periodic x face over j=0..m+1, k=0..n+1 (SRC/bcond.periodic.F90:105-119)
+// periodic x face over j=0..m+1, k=0..n+1 (SRC/bcond.periodic.F90:105-119),
+// one thread per (population, row): q = 0..4 copies direction q+1 from
+// i = l to i = 0, q = 5..9 direction q+5 from i = 1 to i = l+1.  Each value
+// lands in a line of its own, so the face is bound by line transactions,
+// not bytes; a thread per row (ten values each) left one wave per SIMD.
+// Rows are counted in whole waves per population (x_face_rows), so q is the
+// same across a wave and the population pointer is a scalar.
+__host__ __device__ inline long x_face_rows(const Geometry& g) { return ((long)(g.m + 2) * (g.n + 2) + 63) / 64 * 64; }
+
 template <typename R>
 __global__ void periodic_x(Lattice19<R> a, Geometry g)
 {
+    const long rows = (long)(g.m + 2) * (g.n + 2), padded = x_face_rows(g);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = __builtin_amdgcn_readfirstlane((int)(t / padded));
+    const long row = t - (long)q * padded;
+    if (q >= 10 || row >= rows) return;
     const int nj = g.m + 2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nj * (g.n + 2)) return;
-    const int j = t % nj, k = t / nj, l = g.l, l1 = g.l + 1;
-#pragma unroll
-    for (int d = 1; d <= 5; ++d) BGK_A(d, 0, j, k) = BGK_A(d, l, j, k);
-#pragma unroll
-    for (int d = 10; d <= 14; ++d) BGK_A(d, l1, j, k) = BGK_A(d, 1, j, k);
+    const int j = (int)(row % nj), k = (int)(row / nj);
+    if (q < 5)
+        BGK_A(q + 1, 0, j, k) = BGK_A(q + 1, g.l, j, k);
+    else
+        BGK_A(q + 5, g.l + 1, j, k) = BGK_A(q + 5, 1, j, k);
 }
 
 // periodic y face over i=0..l+1, k=0..n+1 (SRC/bcond.periodic.F90:171-185)
