@@ -278,8 +278,7 @@ void bcond_channel_gpu(real_t** a, int l, int m, int n, int nx, int ny)
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
     launch_face(periodic_x<real_t>, 10 * x_face_rows(g), "periodic_x", A, g);
-    launch_face(periodic_y<real_t>, (long)(l + 2) * (n + 2), "periodic_y", A, g);
-    launch_face(walls_z<real_t>, (long)l * m, "walls_z", A, g, real_t(0), 0);
+    launch_face(channel_yz<real_t>, (long)(l + 2) * (n + 2) + (long)l * m, "channel_yz", A, g);
 }
 
 void bcond_driven_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t force)

@@ -105,12 +105,37 @@ __device__ __forceinline__ void walls_z_cell(Lattice19<R>& a, const Geometry& g,
     BGK_A(16, i, j + 1, n1) = BGK_A(7, i, j, n);
 }
 
+// bcond_channel's y faces and z walls in one launch (SRC/bcond.channel.F90,
+// after the x faces): threads [0, (l+2)(n+2)) copy the y faces over
+// i = 0..l+1, k = 0..n+1 as periodic_y does, the last l*m threads write the
+// no-slip walls at z = 0, n+1 (walls_z_cell, lid off).  No cell the y copy reads is
+// written by the walls, and the four populations both write at the same
+// cell (7 and 18 at k = 0, 9 and 16 at k = n+1, for 1 <= i <= l) keep the
+// walls' value, written last in the reference: the y copy skips them there.
 template <typename R>
-__global__ void walls_z(Lattice19<R> a, Geometry g, R force, int lid)
+__global__ void channel_yz(Lattice19<R> a, Geometry g)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.l * g.m) return;
-    walls_z_cell(a, g, 1 + t % g.l, 1 + t / g.l, force, lid);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ni = g.l + 2;
+    const long ny_f = (long)ni * (g.n + 2), nz_f = (long)g.l * g.m;
+    if (t < ny_f) {
+        const int i = (int)(t % ni), k = (int)(t / ni), m = g.m, m1 = g.m + 1;
+        const bool wall_col = i >= 1 && i <= g.l;
+        const bool lo = wall_col && k == 0, hi = wall_col && k == g.n + 1;
+        BGK_A(3, i, 0, k) = BGK_A(3, i, m, k);
+        if (!lo) BGK_A(7, i, 0, k) = BGK_A(7, i, m, k);
+        BGK_A(8, i, 0, k) = BGK_A(8, i, m, k);
+        if (!hi) BGK_A(9, i, 0, k) = BGK_A(9, i, m, k);
+        BGK_A(12, i, 0, k) = BGK_A(12, i, m, k);
+        BGK_A(1, i, m1, k) = BGK_A(1, i, 1, k);
+        BGK_A(10, i, m1, k) = BGK_A(10, i, 1, k);
+        if (!hi) BGK_A(16, i, m1, k) = BGK_A(16, i, 1, k);
+        BGK_A(17, i, m1, k) = BGK_A(17, i, 1, k);
+        if (!lo) BGK_A(18, i, m1, k) = BGK_A(18, i, 1, k);
+    } else if (t < ny_f + nz_f) {
+        const long u = t - ny_f;
+        walls_z_cell(a, g, 1 + (int)(u % g.l), 1 + (int)(u / g.l), R(0), 0);
+    }
 }
 
 // bcond_driven in one launch: threads [0, m*n) take the x faces, the next
