@@ -4,8 +4,8 @@
 ! (SRC/bgk3d.F90:75-246): read the `&parameters` namelist of a bgk.input
 ! file, allocate the 19 + 18 population arrays (here with hipMalloc through
 ! bgk_dev_malloc instead of OpenMP `target data` maps), build f = f_eq on
-! the host, then per step call the boundary routine and col_mc_gpu and swap
-! the a/b pointers (SRC/col_MC.F90:539-596); every icheck steps write the
+! the device (bgk_init_gpu), then per step call the boundary routine and
+! col_mc_gpu and swap the a/b pointers (SRC/col_MC.F90:539-596); every icheck steps write the
 ! diagno.dat line (SRC/diagno.F90:145) from the device reduction.
 !
 ! Usage: bgk3d_host.x BGK_INPUT CASE OUTDIR
@@ -62,25 +62,25 @@
       mode = 0
       if (trim(case) == 'periodic') mode = 1
 
-! f = f_eq on the host, then to the device
+! f = f_eq written straight into both lattices on the device (bit-identical
+! to the reference's host init; sin/cos per coordinate on the host)
       allocate(h(0:nx-1, 0:ny-1, 0:nz-1, 19))
-      h = huge(h)
       do d = 1, 19
          ph(d) = c_loc(h(0,0,0,d))
       enddo
-      rc = bgk_init_host(ph, lx, ly, lz, nx, ny, mode, u00)
-      if (rc /= 0) stop 'bgk_init_host failed'
       do d = 1, 19
          rc = bgk_dev_malloc(pa(d), bytes)
          if (rc /= 0) stop 'hipMalloc failed'
-         rc = bgk_memcpy_h2d(pa(d), ph(d), bytes)
       enddo
       do d = 1, 18
          rc = bgk_dev_malloc(pb(d), bytes)
          if (rc /= 0) stop 'hipMalloc failed'
-         rc = bgk_memcpy_h2d(pb(d), ph(d), bytes)
       enddo
       pb(19) = pa(19)
+      rc = bgk_init_gpu(pa, lx, ly, lz, nx, ny, mode, u00)
+      if (rc /= 0) stop 'bgk_init_gpu failed'
+      rc = bgk_init_gpu(pb, lx, ly, lz, nx, ny, mode, u00)
+      if (rc /= 0) stop 'bgk_init_gpu failed'
 
       open(63, file=trim(outdir)//'/diagno.dat', status='unknown')
       call gpu_device_sync()
