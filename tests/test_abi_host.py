@@ -43,6 +43,20 @@ def test_library_exports_every_declared_symbol(libname, header):
     assert not missing, f"{libname} lacks {missing}"
 
 
+@pytest.mark.parametrize("libname", ["libbgk3d_gpu_sp.so", "libbgk3d_gpu_dp.so"])
+def test_fortran_module_binds_exported_symbols(libname):
+    # every interface of the Fortran module names a symbol the C header
+    # declares and the library exports, so a Fortran host links against it
+    text = (REPO / "bgk3d_gpu_amd" / "fortran" / "bgk3d_gpu_mod.F90").read_text()
+    binds = sorted(set(re.findall(r'bind\(\s*C\s*,\s*name\s*=\s*"(\w+)"', text, re.I)))
+    assert len(binds) >= 10
+    names = set(declared("bgk3d_gpu.h"))
+    lib = ctypes.CDLL(str(rt.LIBDIR / libname))
+    assert [b for b in binds if b not in names] == []
+    assert [b for b in binds if not hasattr(lib, b)] == []
+    assert "bgk_init_gpu" in binds and "col_mc_gpu" in binds
+
+
 def test_reference_symbols_present():
     # exactly the names the reference's Fortran binds (SRC/col_MC.F90:101-123,
     # SRC/bcond_gpu_mod.F90:30-77)
