@@ -39,6 +39,7 @@ class Case:
     les: bool = False              # Smagorinsky LES relaxation (the reference's -DLES builds)
     cte_s: float = CTE_S_REF       # Smagorinsky constant (namelist cteS)
     wall_axis: int = 3             # channel walls: 3 at z = 1, n (the reference), 2 at y = 1, m
+    noshift: bool = False          # unshifted storage, the reference's -DNOSHIFT builds (reference ABI only)
     extra: dict = field(default_factory=dict)
 
     def __post_init__(self):

@@ -137,6 +137,30 @@ __device__ __forceinline__ typename Storage<S>::compute pull_load(const S* ptr)
         return Storage<S>::load(ptr);
 }
 
+// The moving lid: a pull across the top wall adds +force to direction 2
+// and -force to direction 11 (d = 1 and 10 here).  The reference stores that
+// sum in the halo (SRC/bcond.driven.F90:243-247), so it is rounded to the
+// storage type before the collision reads it: exact for FP32 and FP64 with
+// their own arithmetic, the reference's rounding for the mixed forms.  The
+// lean 16-bit forms keep the FP32 sum: rounded to BF16 it moves the lid's
+// momentum input by up to half a unit (0.2 %), and an LES cavity's velocity
+// error against FP64 rose past 5 %.
+template <typename S, int d>
+__device__ __forceinline__ typename Storage<S>::compute lid_value(typename Storage<S>::compute v, bool top,
+                                                                  typename Storage<S>::compute force)
+{
+    if constexpr (d == 1 || d == 10) {
+        if (top) {
+            const typename Storage<S>::compute w = d == 1 ? v + force : v - force;
+            if constexpr (Storage<S>::lean)
+                return w;
+            else
+                return Storage<S>::widen(Rounder<S, 0>::cvt(w, 0u));
+        }
+    }
+    return v;
+}
+
 // EDGE = the row (j,k) touches a y or z boundary; otherwise only the x
 // direction can leave the interior and the j/k checks are compiled out.
 //
@@ -198,10 +222,7 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
             const long src = s + ds + dx;
             const bool b = wall || bounce;
             v = pull_load<S, NT0 ? cx : 1>(b ? (p.a[opp_of(d)] + s) : (p.a[d] + src));
-            if (BZ == AX_LID && EDGE) {
-                if (d == 1 && top) v = v + p.lidforce;
-                if (d == 10 && top) v = v - p.lidforce;
-            }
+            if (BZ == AX_LID && EDGE) v = lid_value<S, d>(v, top, p.lidforce);
             x[d] = v;
             return;
         }
@@ -219,10 +240,7 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
                 v = pull_load<S, NT0 ? cx : 1>(nptr);
             }
         }
-        if (BZ == AX_LID && EDGE) {
-            if (d == 1 && top) v = v + p.lidforce;
-            if (d == 10 && top) v = v - p.lidforce;
-        }
+        if (BZ == AX_LID && EDGE) v = lid_value<S, d>(v, top, p.lidforce);
         x[d] = v;
     });
 }
@@ -347,20 +365,20 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     constexpr bool LES = (FMODE & 4) != 0;
     C fx, fy, fz;
     force_terms<C, (FMODE & 3)>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
-    if constexpr (sizeof(S) == 8 && !LES) {
-        // FP64: reference-order arithmetic, each value stored as soon as it
+    if constexpr (std::is_same<C, double>::value && !LES) {
+        // FP64 arithmetic: reference-order, each value stored as soon as it
         // is formed (77 vector registers instead of 86: 6 waves per SIMD
         // without spilling).  The FP32 form keeps the array collide: with
         // its stores spread through the arithmetic, rows that end in a
         // partly interior line (n = 134 ... 182) ran 18-24 % slower.
         bgk_collide_stream(x, p.omega, fx, fy, fz, p.cst, [&](auto dc, C v) {
             constexpr int d = decltype(dc)::value;
-            store_nt((d < 18 ? p.b[d] : p.a[18]) + (row + i), v);
+            store_nt((d < 18 ? p.b[d] : p.a[18]) + (row + i), Rounder<S, 0>::cvt(v, 0u));
         });
         return;
     }
     C o[19];
-    if constexpr (sizeof(S) == 2)
+    if constexpr (Storage<S>::lean)
         bgk_collide_lean<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
     else
         bgk_collide_ref<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
@@ -388,7 +406,7 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
 template <typename S, int FMODE>
 struct FusedWaves {
     static constexpr bool LES = (FMODE & 4) != 0;
-    static constexpr int value = (!LES && sizeof(S) == 8) ? 6 : 1;
+    static constexpr int value = (!LES && std::is_same<typename Storage<S>::compute, double>::value) ? 6 : 1;
 };
 
 template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>

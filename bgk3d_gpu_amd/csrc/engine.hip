@@ -280,7 +280,7 @@ void one_step(bgk_handle& h, hipEvent_t pre, hipEvent_t post)
                            h.current<S>(), h.geom());
     }
     if (pre) hipEventRecord(pre, h.stream);
-    if (h.cfg.rounding && (h.cfg.storage == BGK_F16 || h.cfg.storage == BGK_BF16))
+    if (h.cfg.rounding && sizeof(S) == 2)
         dispatch_fused<S, 1>(h);
     else
         dispatch_fused<S, 0>(h);
@@ -339,6 +339,25 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
     if (c.boundary == BGK_BC_CHANNEL && c.wall_axis == 2 && c.force_axis == 2)
         return fail(BGK_EINVAL, "walls at y = 1, m: the body force must lie along them (force_axis 1 or 3)");
     if (c.les != 0 && c.les != 1) return fail(BGK_EINVAL, "les %d", c.les);
+    if (c.mixed < 0 || c.mixed > 2) return fail(BGK_EINVAL, "mixed %d", c.mixed);
+    if (c.mixed == 1 && c.storage != BGK_F16) return fail(BGK_EINVAL, "mixed 1 (MIXED1) stores FP16 (storage 2)");
+    if (c.mixed == 2 && c.storage != BGK_F32) return fail(BGK_EINVAL, "mixed 2 (MIXED2) stores FP32 (storage 0)");
+    if (c.obstacle) {
+        // The link masks and the drag/lift mask cover the box centre +-
+        // (radius + 4) clamped to the lattice, without wrapping: along a
+        // periodic axis the box must lie inside, or cells across the seam
+        // would pull from solid cells without bouncing back.
+        const int ctr[3] = {c.obs_ic >= 0 ? c.obs_ic : 2 * c.l / 5, c.obs_jc >= 0 ? c.obs_jc : c.m / 2,
+                            c.obs_kc >= 0 ? c.obs_kc : c.n / 2};
+        const int R = c.obs_radius >= 0 ? c.obs_radius : (c.m > 256 ? 32 : c.m / 8);
+        const int size[3] = {c.l, c.m, c.n};
+        for (int ax = c.boundary == BGK_BC_PERIODIC ? 0 : 1; ax < 3; ++ax)
+            if (ctr[ax] - (R + 4) < 1 || ctr[ax] + (R + 4) > size[ax])
+                return fail(BGK_EINVAL,
+                            "obstacle box %d +- %d leaves 1..%d along periodic axis %c: move the sphere away from "
+                            "the seam",
+                            ctr[ax], R + 4, size[ax], "xyz"[ax]);
+    }
     h.cfg = c;
     h.esize = storage_bytes(c.storage);
     h.A = (int)(128 / h.esize);
@@ -351,7 +370,7 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
     h.lstride = round_up(h.A + h.nxy * h.nz, h.A);
     // derived parameters in the collision kind (SRC/hencol.F90:31, :52;
     // SRC/bcond.driven.F90:61; SRC/build.bcond.F90:29-35)
-    const bool dbl = c.storage == BGK_F64;
+    const bool dbl = compute_double(c);
     auto K = [&](double v) { return dbl ? v : (double)(float)v; };
     const double nu = K(c.svisc), u0f = K(c.boundary == BGK_BC_CHANNEL ? c.u0 : 0.0),
                  u00 = K(c.boundary == BGK_BC_CHANNEL ? 0.0 : c.u0);
@@ -392,6 +411,7 @@ void bgk_config_default(bgk_config* c, int l, int m, int n)
     c->les = 0;
     c->cte_s = -1.0;
     c->wall_axis = 3;
+    c->mixed = 0;
 }
 
 const char* bgk_last_error(void) { return last_error(); }
@@ -487,7 +507,7 @@ int bgk_create(const bgk_config* cfg, bgk_handle** out)
         hipHostMalloc(&h->hscratch, (kRedBlocks + 8) * 5 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(BGK_ENOMEM, "scratch"));
     if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(fail(BGK_EHIP, "memset"));
-    rc = by_storage(c.storage, [&](auto tag) { return upload_state<decltype(tag)>(*h); });
+    rc = by_storage(c, [&](auto tag) { return upload_state<decltype(tag)>(*h); });
     if (rc) return cleanup(rc);
     *out = h;
     return 0;
@@ -514,7 +534,7 @@ int bgk_step(bgk_handle* h, int nsteps)
 {
     if (!h || nsteps < 0) return fail(BGK_EINVAL, "bad arguments");
     ENG_HIP(hipSetDevice(h->cfg.device));
-    return by_storage(h->cfg.storage, [&](auto tag) {
+    return by_storage(h->cfg, [&](auto tag) {
         for (int s = 0; s < nsteps; ++s) one_step<decltype(tag)>(*h, nullptr, nullptr);
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : fail(BGK_EHIP, "step launch: %s", hipGetErrorString(e));
@@ -541,7 +561,7 @@ int bgk_time_steps(bgk_handle* h, int nsteps, double* total_ms, double* collide_
     ENG_HIP(hipStreamSynchronize(h->stream));
     hipEvent_t e0 = h->events[0], e1 = h->events[1];
     ENG_HIP(hipEventRecord(e0, h->stream));
-    int rc = by_storage(h->cfg.storage, [&](auto tag) {
+    int rc = by_storage(h->cfg, [&](auto tag) {
         for (int s = 0; s < nsteps; ++s)
             one_step<decltype(tag)>(*h, h->events[2 + 2 * s], h->events[3 + 2 * s]);
         return 0;
@@ -568,7 +588,7 @@ int bgk_diagno(bgk_handle* h, double out[5])
     ENG_HIP(hipSetDevice(h->cfg.device));
     const long rows = (long)h->cfg.m * h->cfg.n;
     const int nb = (int)std::min<long>(rows, kRedBlocks);
-    int rc = by_storage(h->cfg.storage, [&](auto tag) {
+    int rc = by_storage(h->cfg, [&](auto tag) {
         typedef decltype(tag) S;
         hipLaunchKernelGGL(diagno_partial<S>, dim3(nb), dim3(kRedThreads), 0, h->stream, h->current<S>(), h->geom(),
                            cte_one<S>(), h->scratch + 5);
@@ -596,7 +616,7 @@ int bgk_draglift(bgk_handle* h, double out[3])
     const int ks = std::max(1, h->kc - h->radius - delta), ke = std::min(c.n, h->kc + h->radius + delta);
     const long cells = (long)(ie - is + 1) * (je - js + 1) * (ke - ks + 1);
     const int nb = (int)std::min<long>((cells + kRedThreads - 1) / kRedThreads, kRedBlocks);
-    by_storage(c.storage, [&](auto tag) {
+    by_storage(c, [&](auto tag) {
         typedef decltype(tag) S;
         hipLaunchKernelGGL(draglift_box<S>, dim3(nb), dim3(kRedThreads), 0, h->stream, h->current<S>(), h->geom(),
                            h->omask, h->mk[0], h->mk[1], h->mk[2], h->mk[3], h->mk[4], h->mk[5], is, ie, js, je, ks,
@@ -609,7 +629,7 @@ int bgk_draglift(bgk_handle* h, double out[3])
     ENG_HIP(hipStreamSynchronize(h->stream));
     // norm = uno/(u_inflow*u_inflow*radius) in the working kind (SRC/draglift.F90:178)
     double norm;
-    if (c.storage == BGK_F64)
+    if (compute_double(c))
         norm = 1.0 / (h->u_inflow * h->u_inflow * (double)h->radius);
     else {
         const float u = (float)h->u_inflow;
@@ -626,7 +646,7 @@ int bgk_fields(bgk_handle* h, double* rho, double* ux, double* uy, double* uz)
     const long cells = (long)h->cfg.l * h->cfg.m * h->cfg.n;
     double* d = nullptr;
     ENG_HIP(hipMalloc(&d, 4 * cells * sizeof(double)));
-    by_storage(h->cfg.storage, [&](auto tag) {
+    by_storage(h->cfg, [&](auto tag) {
         typedef decltype(tag) S;
         hipLaunchKernelGGL(fields_kernel<S>, dim3(ceil_div(cells, 256)), dim3(256), 0, h->stream, h->current<S>(),
                            h->geom(), cte_one<S>(), d, d + cells, d + 2 * cells, d + 3 * cells);
@@ -652,7 +672,7 @@ int bgk_profile(bgk_handle* h, int axis, int c1, int c2, double* out)
     if (c1 < 1 || c1 > lim1 || c2 < 1 || c2 > lim2)
         return fail(BGK_EINVAL, "line (%d,%d) outside the interior", c1, c2);
     ENG_HIP(hipSetDevice(c.device));
-    return by_storage(c.storage, [&](auto tag) {
+    return by_storage(c, [&](auto tag) {
         typedef decltype(tag) S;
         typedef typename Storage<S>::compute C;
         C* tmp = nullptr;
@@ -680,7 +700,7 @@ int bgk_probe(bgk_handle* h, int i0, int j0, int k0, double out[4])
     if (i0 < 1 || i0 > c.l || j0 < 1 || j0 > c.m || k0 < 1 || k0 > c.n)
         return fail(BGK_EINVAL, "point (%d,%d,%d) outside the interior", i0, j0, k0);
     ENG_HIP(hipSetDevice(c.device));
-    return by_storage(c.storage, [&](auto tag) {
+    return by_storage(c, [&](auto tag) {
         typedef decltype(tag) S;
         typedef typename Storage<S>::compute C;
         C* tmp = nullptr;
@@ -705,7 +725,7 @@ int bgk_get_population(bgk_handle* h, int dir, double* out)
     const long cells = (long)h->cfg.l * h->cfg.m * h->cfg.n;
     double* d = nullptr;
     ENG_HIP(hipMalloc(&d, cells * sizeof(double)));
-    by_storage(h->cfg.storage, [&](auto tag) {
+    by_storage(h->cfg, [&](auto tag) {
         typedef decltype(tag) S;
         hipLaunchKernelGGL(get_interior<S>, dim3(ceil_div(cells, 256)), dim3(256), 0, h->stream, h->cur_a<S>(dir),
                            h->geom(), d);
@@ -730,7 +750,7 @@ int bgk_set_population(bgk_handle* h, int dir, const double* in)
     if (hipMemcpyAsync(d, in, cells * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
         rc = fail(BGK_EHIP, "population upload");
     if (rc == 0)
-        by_storage(h->cfg.storage, [&](auto tag) {
+        by_storage(h->cfg, [&](auto tag) {
             typedef decltype(tag) S;
             hipLaunchKernelGGL(set_interior<S>, dim3(ceil_div(cells, 256)), dim3(256), 0, h->stream, h->cur_a<S>(dir),
                                h->geom(), d);
@@ -758,7 +778,7 @@ int bgk_save_raw(bgk_handle* h, const char* path, int itime)
     int rc = hipMalloc(&dev, bytes) == hipSuccess ? 0 : fail(BGK_ENOMEM, "save_raw staging");
     if (rc == 0) rc = write_record(f, &itime, 4);
     for (int d = 0; d < 19 && rc == 0; ++d) {
-        rc = by_storage(c.storage, [&](auto tag) {
+        rc = by_storage(c, [&](auto tag) {
             typedef decltype(tag) S;
             hipLaunchKernelGGL(pack_halo<S>, dim3(ceil_div(cnt, 256)), dim3(256), 0, h->stream, h->cur_a<S>(d),
                                h->geom(), static_cast<S*>(dev));
@@ -795,7 +815,7 @@ int bgk_restore_raw(bgk_handle* h, const char* path, int* itime)
             rc = fail(BGK_EHIP, "restore_raw upload");
             break;
         }
-        rc = by_storage(c.storage, [&](auto tag) {
+        rc = by_storage(c, [&](auto tag) {
             typedef decltype(tag) S;
             hipLaunchKernelGGL(unpack_halo<S>, dim3(ceil_div(cnt, 256)), dim3(256), 0, h->stream,
                                static_cast<const S*>(dev), h->geom(), h->cur_a<S>(d));
@@ -808,7 +828,7 @@ int bgk_restore_raw(bgk_handle* h, const char* path, int* itime)
     if (rc) return rc;
     // the engine's own constant inflow face (both lattices)
     if (c.boundary == BGK_BC_INFLOW) {
-        rc = by_storage(c.storage, [&](auto tag) {
+        rc = by_storage(c, [&](auto tag) {
             typedef decltype(tag) S;
             typedef typename Storage<S>::compute C;
             for (int buf = 0; buf < 2; ++buf) {
@@ -838,7 +858,7 @@ int bgk_save_vel(bgk_handle* h, const char* path, int itime)
     int rc = write_record(f, &itime, 4);
     // one record per cell, three values of the collision kind (i fastest)
     for (long t = 0; t < cells && rc == 0; ++t) {
-        if (c.storage == BGK_F64) {
+        if (compute_double(c)) {
             const double v[3] = {ux[t], uy[t], uz[t]};
             rc = write_record(f, v, sizeof v);
         } else {
@@ -873,7 +893,7 @@ int bgk_write_vtk(bgk_handle* h, int kind, int index, const char* path)
     npts = (long)dims[0] * dims[1] * dims[2];
     float* dev = nullptr;
     ENG_HIP(hipMalloc(&dev, (size_t)npts * 3 * sizeof(float)));
-    by_storage(c.storage, [&](auto tag) {
+    by_storage(c, [&](auto tag) {
         typedef decltype(tag) S;
         if (kind == 3)
             hipLaunchKernelGGL(vorticity_xy_kernel<S>, dim3(ceil_div(npts, 256)), dim3(256), 0, h->stream,

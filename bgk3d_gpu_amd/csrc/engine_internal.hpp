@@ -101,17 +101,25 @@ inline size_t storage_bytes(int kind)
 
 inline long round_up(long v, long a) { return (v + a - 1) / a * a; }
 
+// Calls f with a value of the storage type the configuration selects: the
+// storage kind, or for the mixed forms the type that carries their
+// collision kind (mix32_t, f16r_t).
 template <typename F>
-int by_storage(int kind, F&& f)
+int by_storage(const bgk_config& c, F&& f)
 {
-    switch (kind) {
+    if (c.mixed == 2 && c.storage == BGK_F32) return f(mix32_t{});
+    if (c.mixed == 1 && c.storage == BGK_F16) return f(f16r_t{});
+    switch (c.storage) {
         case BGK_F32: return f(float{});
         case BGK_F64: return f(double{});
         case BGK_F16: return f(f16_t{});
         case BGK_BF16: return f(bf16_t{});
     }
-    return fail(BGK_EINVAL, "unknown storage type %d", kind);
+    return fail(BGK_EINVAL, "unknown storage type %d", c.storage);
 }
+
+// The collision runs in FP64 (DOUBLE_P storage, or the MIXED2 form).
+inline bool compute_double(const bgk_config& c) { return c.storage == BGK_F64 || c.mixed == 2; }
 
 // Validate `c` and fill the layout and the derived parameters of `h`
 // (no device call, no allocation).

@@ -1,7 +1,9 @@
 // Host-side initial state f = f_eq (shifted storage), shared by the
 // reference-compatible ABI and the handle API.
 //
-// SRC/init.F90:31-209 with cte1 = 0 (shifted storage, :71-75):
+// SRC/init.F90:31-209 with cte1 = 0 (shifted storage, :71-75; with the flag
+// kInitUnshifted in `mode`, cte1 = 1: the populations themselves, as the
+// reference's -DNOSHIFT builds store them):
 //   mode 0: rest flow -> every population is 0;
 //   mode 1: 3-D Taylor-Green, u = u00 sin x cos y cos z, v = -u00 cos x
 //           sin y cos z, w = 0, crho = 1 + u00^2/48 (cos 2x + cos 2y)
@@ -26,6 +28,10 @@
 #include "d3q19.hpp"
 
 namespace bgk {
+
+// Flag of the init mode: unshifted storage (f instead of f - w), the
+// reference's -DNOSHIFT convention (SRC/init.F90:71-75).  Modes 0 and 1 only.
+constexpr int kInitUnshifted = 8;
 
 // Trigonometric factors of one coordinate as the reference forms them
 // (SRC/init.F90:115-123, 142-148): the coordinate (i - 1/2)/l rounded to the
@@ -54,7 +60,9 @@ template <typename R>
 __host__ __device__ inline void feq_cell(int mode, R u00, const TgAxis<R>& X, const TgAxis<R>& Y, const TgAxis<R>& Z,
                                          R (&h)[19])
 {
-    const R uno = 1, due = 2, tre = 3, qua = 4, zero = 0, rf = 3, qf = R(1.5), cte1 = 0;
+    const R uno = 1, due = 2, tre = 3, qua = 4, zero = 0, rf = 3, qf = R(1.5);
+    const R cte1 = (mode & kInitUnshifted) ? uno : zero;
+    mode &= ~kInitUnshifted;
     const R w[19] = {W<R>::p2, W<R>::p2, W<R>::p2, W<R>::p2, W<R>::p1, W<R>::p1, W<R>::p2,
                      W<R>::p1, W<R>::p2, W<R>::p2, W<R>::p2, W<R>::p2, W<R>::p2, W<R>::p1,
                      W<R>::p1, W<R>::p2, W<R>::p1, W<R>::p2, W<R>::p0};
@@ -83,7 +91,7 @@ __host__ __device__ inline void feq_cell(int mode, R u00, const TgAxis<R>& X, co
 template <typename R>
 int init_feq_host(R** h, int l, int m, int n, int nx, int ny, int mode, R u00, int k0 = 0, int k1 = -1)
 {
-    if (mode < 0 || mode > 2) return -1;
+    if (mode < 0 || (mode & ~kInitUnshifted) > 2 || mode == (2 | kInitUnshifted)) return -1;
     if (k1 < 0) k1 = n + 1;
     if (k0 < 0 || k1 > n + 1 || k0 > k1) return -1;
     const size_t nxy = (size_t)nx * (size_t)ny;

@@ -1,9 +1,13 @@
 // Population storage types and their conversion to the collision kind.
 //
-// float  -> float   (reference SINGLE)
-// double -> double  (reference DOUBLE_P)
-// f16    -> float   (_Float16 storage of f - w, FP32 collision)
-// bf16   -> float   (bfloat16 storage of f - w, FP32 collision)
+// float   -> float   (reference SINGLE)
+// double  -> double  (reference DOUBLE_P)
+// f16     -> float   (_Float16 storage of f - w, FP32 collision, lean form)
+// bf16    -> float   (bfloat16 storage of f - w, FP32 collision, lean form)
+// mix32_t -> double  (FP32 storage, FP64 collision: the reference's MIXED2
+//                     build, SRC/precision.F90:38-43)
+// f16r_t  -> float   (FP16 storage, FP32 collision in the reference's
+//                     expression order: its MIXED1 build, :30-36)
 //
 // Stored values are the reference's shifted populations f_i - w_i
 // (SRC/col_MC.F90:139-144), which keeps them near zero where low-precision
@@ -23,9 +27,21 @@ struct bf16_t {
     uint16_t u;
 };
 typedef _Float16 f16_t;
+// Types of their own for the mixed forms, so the kernels select the
+// collision kind and form at compile time; the bits stored are a float and
+// a _Float16.
+struct mix32_t {
+    float v;
+};
+struct f16r_t {
+    _Float16 v;
+};
 
-enum StorageKind : int { ST_F32 = 0, ST_F64 = 1, ST_F16 = 2, ST_BF16 = 3 };
+enum StorageKind : int { ST_F32 = 0, ST_F64 = 1, ST_F16 = 2, ST_BF16 = 3, ST_MIX32 = 4, ST_F16R = 5 };
 
+// load / load_nt: a stored value in the collision kind (plain or streaming
+// read); widen: the same for a value in a register.  lean: the 16-bit lean
+// collide (bgk_collide_lean) instead of the reference-order one.
 template <typename S>
 struct Storage;
 
@@ -33,27 +49,34 @@ template <>
 struct Storage<float> {
     typedef float compute;
     static constexpr int kind = ST_F32;
+    static constexpr bool lean = false;
     __device__ __forceinline__ static float load(const float* p) { return *p; }
     __device__ __forceinline__ static float load_nt(const float* p) { return __builtin_nontemporal_load(p); }
+    __device__ __forceinline__ static float widen(float v) { return v; }
 };
 template <>
 struct Storage<double> {
     typedef double compute;
     static constexpr int kind = ST_F64;
+    static constexpr bool lean = false;
     __device__ __forceinline__ static double load(const double* p) { return *p; }
     __device__ __forceinline__ static double load_nt(const double* p) { return __builtin_nontemporal_load(p); }
+    __device__ __forceinline__ static double widen(double v) { return v; }
 };
 template <>
 struct Storage<f16_t> {
     typedef float compute;
     static constexpr int kind = ST_F16;
+    static constexpr bool lean = true;
     __device__ __forceinline__ static float load(const f16_t* p) { return (float)*p; }
     __device__ __forceinline__ static float load_nt(const f16_t* p) { return (float)__builtin_nontemporal_load(p); }
+    __device__ __forceinline__ static float widen(f16_t v) { return (float)v; }
 };
 template <>
 struct Storage<bf16_t> {
     typedef float compute;
     static constexpr int kind = ST_BF16;
+    static constexpr bool lean = true;
     __device__ __forceinline__ static float load(const bf16_t* p)
     {
         return __uint_as_float((uint32_t)p->u << 16);
@@ -62,6 +85,31 @@ struct Storage<bf16_t> {
     {
         return __uint_as_float((uint32_t)__builtin_nontemporal_load(&p->u) << 16);
     }
+    __device__ __forceinline__ static float widen(bf16_t v) { return __uint_as_float((uint32_t)v.u << 16); }
+};
+template <>
+struct Storage<mix32_t> {
+    typedef double compute;
+    static constexpr int kind = ST_MIX32;
+    static constexpr bool lean = false;
+    __device__ __forceinline__ static double load(const mix32_t* p) { return (double)p->v; }
+    __device__ __forceinline__ static double load_nt(const mix32_t* p)
+    {
+        return (double)__builtin_nontemporal_load(&p->v);
+    }
+    __device__ __forceinline__ static double widen(mix32_t v) { return (double)v.v; }
+};
+template <>
+struct Storage<f16r_t> {
+    typedef float compute;
+    static constexpr int kind = ST_F16R;
+    static constexpr bool lean = false;
+    __device__ __forceinline__ static float load(const f16r_t* p) { return (float)p->v; }
+    __device__ __forceinline__ static float load_nt(const f16r_t* p)
+    {
+        return (float)__builtin_nontemporal_load(&p->v);
+    }
+    __device__ __forceinline__ static float widen(f16r_t v) { return (float)v.v; }
 };
 
 __host__ __device__ inline uint32_t hash32(uint32_t x)
@@ -82,8 +130,8 @@ __host__ __device__ inline uint32_t hash32(uint32_t x)
 // 32-bit multiply per store).
 __device__ __forceinline__ uint32_t sr_word(uint32_t r0, int d) { return r0 + (uint32_t)(d + 1) * 0x9e3779b9u; }
 
-// Round a float to the storage type.  SR = 0: nearest-even; SR = 1:
-// stochastic with the 32-bit random word r.
+// Round a value of the collision kind to the storage type.  SR = 0:
+// nearest-even; SR = 1: stochastic with the 32-bit random word r.
 template <typename S, int SR>
 struct Rounder;
 
@@ -133,7 +181,23 @@ struct Rounder<bf16_t, 1> {
     }
 };
 
-// Stores (plain and non-temporal); bf16_t is stored through its bits.
+// FP64 to FP32 with nearest-even rounding (the reference's assignment of a
+// double expression to a single-precision array element).
+template <int SR>
+struct Rounder<mix32_t, SR> {
+    __device__ __forceinline__ static mix32_t cvt(double v, uint32_t) { return mix32_t{(float)v}; }
+};
+template <>
+struct Rounder<f16r_t, 0> {
+    __device__ __forceinline__ static f16r_t cvt(float v, uint32_t) { return f16r_t{(f16_t)v}; }
+};
+template <>
+struct Rounder<f16r_t, 1> {
+    __device__ __forceinline__ static f16r_t cvt(float v, uint32_t r) { return f16r_t{Rounder<f16_t, 1>::cvt(v, r)}; }
+};
+
+// Stores (plain and non-temporal); bf16_t and the mixed types are stored
+// through their member.
 template <typename S>
 __device__ __forceinline__ void store_value(S* p, S v) { *p = v; }
 template <>
@@ -145,5 +209,13 @@ __device__ __forceinline__ void store_nt<bf16_t>(bf16_t* p, bf16_t v)
 {
     __builtin_nontemporal_store(v.u, reinterpret_cast<uint16_t*>(p));
 }
+template <>
+__device__ __forceinline__ void store_value<mix32_t>(mix32_t* p, mix32_t v) { p->v = v.v; }
+template <>
+__device__ __forceinline__ void store_nt<mix32_t>(mix32_t* p, mix32_t v) { __builtin_nontemporal_store(v.v, &p->v); }
+template <>
+__device__ __forceinline__ void store_value<f16r_t>(f16r_t* p, f16r_t v) { p->v = v.v; }
+template <>
+__device__ __forceinline__ void store_nt<f16r_t>(f16r_t* p, f16r_t v) { __builtin_nontemporal_store(v.v, &p->v); }
 
 }  // namespace bgk

@@ -267,9 +267,7 @@ void bcond_periodic_gpu(real_t** a, int l, int m, int n, int nx, int ny)
     if (!shape_ok("bcond_periodic_gpu", l, m, n, nx, ny, a)) return;
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
-    launch_face(periodic_x<real_t>, 10 * x_face_rows(g), "periodic_x", A, g);
-    launch_face(periodic_y<real_t>, (long)(l + 2) * (n + 2), "periodic_y", A, g);
-    launch_face(periodic_z<real_t>, (long)(l + 2) * (m + 2), "periodic_z", A, g);
+    launch_face(periodic_faces<real_t, true>, periodic_faces_threads<real_t, true>(g), "periodic_faces", A, g);
 }
 
 void bcond_channel_gpu(real_t** a, int l, int m, int n, int nx, int ny)
@@ -277,8 +275,7 @@ void bcond_channel_gpu(real_t** a, int l, int m, int n, int nx, int ny)
     if (!shape_ok("bcond_channel_gpu", l, m, n, nx, ny, a)) return;
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
-    launch_face(periodic_x<real_t>, 10 * x_face_rows(g), "periodic_x", A, g);
-    launch_face(channel_yz<real_t>, (long)(l + 2) * (n + 2) + (long)l * m, "channel_yz", A, g);
+    launch_face(channel_faces<real_t>, channel_faces_threads(g), "channel_faces", A, g);
 }
 
 void bcond_driven_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t force)
@@ -297,8 +294,7 @@ void bcond_inflow_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t u_
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
     launch_face(inflow_x<real_t>, (long)(m + 2) * (n + 2), "inflow_x", A, g, u_inflow, rf, qf, p1, p2, cte1);
-    launch_face(periodic_y<real_t>, (long)(l + 2) * (n + 2), "periodic_y", A, g);
-    launch_face(periodic_z<real_t>, (long)(l + 2) * (m + 2), "periodic_z", A, g);
+    launch_face(periodic_faces<real_t, false>, periodic_faces_threads<real_t, false>(g), "periodic_yz", A, g);
 }
 
 void bcond_obs_gpu(real_t** a, const int* obs, int l, int m, int n, int nx, int ny, int imin, int imax, int jmin,
@@ -468,8 +464,11 @@ int bgk_init_host(real_t** h, int l, int m, int n, int nx, int ny, int mode, rea
 int bgk_init_gpu(real_t** a, int l, int m, int n, int nx, int ny, int mode, real_t u00)
 {
     if (!shape_ok("bgk_init_gpu", l, m, n, nx, ny, a)) return -1;
-    if (mode < 0 || mode > 2) {
-        std::fprintf(stderr, "bgk3d_gpu: bgk_init_gpu: init mode %d not supported (0, 1, 2)\n", mode);
+    if (mode < 0 || (mode & ~kInitUnshifted) > 2 || mode == (2 | kInitUnshifted)) {
+        std::fprintf(stderr,
+                     "bgk3d_gpu: bgk_init_gpu: init mode %d not supported (0, 1, 2; 0 or 1 plus %d for "
+                     "unshifted storage)\n",
+                     mode, kInitUnshifted);
         return -1;
     }
     const hipError_t e = bgk::init_feq_device<real_t>(a, l, m, n, nx, (long)nx * ny, mode, u00, abi_stream());

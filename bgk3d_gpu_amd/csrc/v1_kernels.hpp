@@ -105,39 +105,6 @@ __device__ __forceinline__ void walls_z_cell(Lattice19<R>& a, const Geometry& g,
     BGK_A(16, i, j + 1, n1) = BGK_A(7, i, j, n);
 }
 
-// bcond_channel's y faces and z walls in one launch (SRC/bcond.channel.F90,
-// after the x faces): threads [0, (l+2)(n+2)) copy the y faces over
-// i = 0..l+1, k = 0..n+1 as periodic_y does, the last l*m threads write the
-// no-slip walls at z = 0, n+1 (walls_z_cell, lid off).  No cell the y copy reads is
-// written by the walls, and the four populations both write at the same
-// cell (7 and 18 at k = 0, 9 and 16 at k = n+1, for 1 <= i <= l) keep the
-// walls' value, written last in the reference: the y copy skips them there.
-template <typename R>
-__global__ void channel_yz(Lattice19<R> a, Geometry g)
-{
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ni = g.l + 2;
-    const long ny_f = (long)ni * (g.n + 2), nz_f = (long)g.l * g.m;
-    if (t < ny_f) {
-        const int i = (int)(t % ni), k = (int)(t / ni), m = g.m, m1 = g.m + 1;
-        const bool wall_col = i >= 1 && i <= g.l;
-        const bool lo = wall_col && k == 0, hi = wall_col && k == g.n + 1;
-        BGK_A(3, i, 0, k) = BGK_A(3, i, m, k);
-        if (!lo) BGK_A(7, i, 0, k) = BGK_A(7, i, m, k);
-        BGK_A(8, i, 0, k) = BGK_A(8, i, m, k);
-        if (!hi) BGK_A(9, i, 0, k) = BGK_A(9, i, m, k);
-        BGK_A(12, i, 0, k) = BGK_A(12, i, m, k);
-        BGK_A(1, i, m1, k) = BGK_A(1, i, 1, k);
-        BGK_A(10, i, m1, k) = BGK_A(10, i, 1, k);
-        if (!hi) BGK_A(16, i, m1, k) = BGK_A(16, i, 1, k);
-        BGK_A(17, i, m1, k) = BGK_A(17, i, 1, k);
-        if (!lo) BGK_A(18, i, m1, k) = BGK_A(18, i, 1, k);
-    } else if (t < ny_f + nz_f) {
-        const long u = t - ny_f;
-        walls_z_cell(a, g, 1 + (int)(u % g.l), 1 + (int)(u / g.l), R(0), 0);
-    }
-}
-
 // bcond_driven in one launch: threads [0, m*n) take the x faces, the next
 // l*n the y faces, the last l*m the z faces with the lid.  Every value read
 // is an interior cell, which no face writes, so the faces interact only
@@ -160,69 +127,171 @@ __global__ void driven_faces(Lattice19<R> a, Geometry g, R force)
     }
 }
 
-// periodic x face over j=0..m+1, k=0..n+1 (SRC/bcond.periodic.F90:105-119),
-// one thread per (population, row): q = 0..4 copies direction q+1 from
-// i = l to i = 0, q = 5..9 direction q+5 from i = 1 to i = l+1.  Each value
-// lands in a line of its own, so the face is bound by line transactions,
-// not bytes; a thread per row (ten values each) left one wave per SIMD.
-// Rows are counted in whole waves per population (x_face_rows), so q is the
-// same across a wave and the population pointer is a scalar.
+// ---- periodic faces in one launch -------------------------------------
+// The reference fills the periodic faces one after the other, each over the
+// whole face including the halo rows of the other axes: x over j = 0..m+1,
+// k = 0..n+1, then y over i = 0..l+1, k = 0..n+1, then z over the full
+// (i, j) face (SRC/bcond.periodic.F90:105-252).  A face writes population d
+// at a halo cell when d enters the interior across it (c = +1 on the low
+// side, c = -1 on the high side) and copies it from the opposite interior
+// layer, reading values earlier faces may have just written.  Followed
+// through, the last value of population d at halo cell h is the old value at
+// h wrapped along every axis on which h lies in the halo and d enters, and
+// it is written only if there is such an axis.  Each (cell, population) is
+// therefore written by exactly one thread here -- the one of the last face
+// in the order x, y, z that writes it -- and every value read lies, on each
+// axis, either inside or on a halo side the population does not enter, so
+// no thread of the launch writes it: one launch gives the three sequential
+// fills' result, edges and corners included.
+
+// -1: low halo (coordinate 0), +1: high halo (max + 1), 0: inside
+__device__ __forceinline__ int halo_side(int c, int max) { return c == 0 ? -1 : (c == max + 1 ? 1 : 0); }
+// population with velocity component c enters the interior across that side
+__device__ __forceinline__ bool enters(int side, int c) { return (side < 0 && c > 0) || (side > 0 && c < 0); }
+__device__ __forceinline__ int wrap_to(int side, int max) { return side < 0 ? max : 1; }
+
+// The populations entering across each face, 0-based (direction - 1):
+// x low / high, y low / high, z low / high.
+__device__ constexpr int kEnterXlo[5] = {0, 1, 2, 3, 4}, kEnterXhi[5] = {9, 10, 11, 12, 13};
+__device__ constexpr int kEnterYlo[5] = {2, 6, 7, 8, 11}, kEnterYhi[5] = {0, 9, 15, 16, 17};
+__device__ constexpr int kEnterZlo[5] = {3, 5, 6, 12, 17}, kEnterZhi[5] = {1, 8, 10, 14, 15};
+
+// Rows of the x face counted in whole waves per population, so that the
+// population index q is the same across a wave.
 __host__ __device__ inline long x_face_rows(const Geometry& g) { return ((long)(g.m + 2) * (g.n + 2) + 63) / 64 * 64; }
 
-template <typename R>
-__global__ void periodic_x(Lattice19<R> a, Geometry g)
+// x face, one thread per (population, row): q = 0..4 sets direction q+1 at
+// i = 0 from i = l, q = 5..9 direction q+5 at i = l+1 from i = 1.  Each value
+// lands in a line of its own, so this part is bound by line transactions.
+// WALLS: the channel's no-slip walls at z = 0, n+1 follow the faces and
+// overwrite population d at (i - c_x, j - c_y) of a wall plane for every
+// interior column (i, j) (walls_z_cell); periodic z otherwise.
+template <typename R, bool WALLS>
+__device__ __forceinline__ void x_face_part(Lattice19<R>& a, const Geometry& g, long t)
 {
     const long rows = (long)(g.m + 2) * (g.n + 2), padded = x_face_rows(g);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int q = __builtin_amdgcn_readfirstlane((int)(t / padded));
     const long row = t - (long)q * padded;
     if (q >= 10 || row >= rows) return;
     const int nj = g.m + 2;
     const int j = (int)(row % nj), k = (int)(row / nj);
-    if (q < 5)
-        BGK_A(q + 1, 0, j, k) = BGK_A(q + 1, g.l, j, k);
-    else
-        BGK_A(q + 5, g.l + 1, j, k) = BGK_A(q + 5, 1, j, k);
+    const int d = q < 5 ? kEnterXlo[q] : kEnterXhi[q - 5];
+    const int sj = halo_side(j, g.m), sk = halo_side(k, g.n);
+    if (enters(sj, cy_of(d))) return;  // the y face writes it later
+    if (enters(sk, cz_of(d))) {
+        if (!WALLS) return;                                   // the z face writes it later
+        if (j + cy_of(d) >= 1 && j + cy_of(d) <= g.m) return;  // a wall writes it later
+    }
+    const int it = q < 5 ? 0 : g.l + 1, is = q < 5 ? g.l : 1;
+    a.p[d][gidx(g, it, j, k)] = a.p[d][gidx(g, is, j, k)];
 }
 
-// periodic y face over i=0..l+1, k=0..n+1 (SRC/bcond.periodic.F90:171-185)
-template <typename R>
-__global__ void periodic_y(Lattice19<R> a, Geometry g)
+// y face at (i, k), i = 0..l+1, k = 0..n+1: directions entering at j = 0
+// (from j = m) and at j = m+1 (from j = 1), the source wrapped in x where
+// the cell is an x halo cell the direction enters and the x face is periodic
+// (XP); without it (bcond_inflow) the x halo already holds its final values
+// and is copied as it is.
+template <typename R, bool WALLS, bool XP = true>
+__device__ __forceinline__ void y_face_part(Lattice19<R>& a, const Geometry& g, int i, int k)
 {
-    const int ni = g.l + 2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ni * (g.n + 2)) return;
-    const int i = t % ni, k = t / ni, m = g.m, m1 = g.m + 1;
-    BGK_A(3, i, 0, k) = BGK_A(3, i, m, k);
-    BGK_A(7, i, 0, k) = BGK_A(7, i, m, k);
-    BGK_A(8, i, 0, k) = BGK_A(8, i, m, k);
-    BGK_A(9, i, 0, k) = BGK_A(9, i, m, k);
-    BGK_A(12, i, 0, k) = BGK_A(12, i, m, k);
-    BGK_A(1, i, m1, k) = BGK_A(1, i, 1, k);
-    BGK_A(10, i, m1, k) = BGK_A(10, i, 1, k);
-    BGK_A(16, i, m1, k) = BGK_A(16, i, 1, k);
-    BGK_A(17, i, m1, k) = BGK_A(17, i, 1, k);
-    BGK_A(18, i, m1, k) = BGK_A(18, i, 1, k);
+    const int si = halo_side(i, g.l), sk = halo_side(k, g.n);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int jt = h == 0 ? 0 : g.m + 1, js = h == 0 ? g.m : 1;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const int d = h == 0 ? kEnterYlo[q] : kEnterYhi[q];
+            if (enters(sk, cz_of(d))) {
+                if (!WALLS) continue;                                 // the z face writes it later
+                if (i + cx_of(d) >= 1 && i + cx_of(d) <= g.l) continue;  // a wall writes it later
+            }
+            const int is = XP && enters(si, cx_of(d)) ? wrap_to(si, g.l) : i;
+            a.p[d][gidx(g, i, jt, k)] = a.p[d][gidx(g, is, js, k)];
+        }
+    }
 }
 
-// periodic z face over i=0..l+1, j=0..m+1 (SRC/bcond.periodic.F90:238-252)
-template <typename R>
-__global__ void periodic_z(Lattice19<R> a, Geometry g)
+// z face at (i, j) over the full face (periodic only), sources wrapped in x
+// (if XP) and y as above.
+template <typename R, bool XP = true>
+__device__ __forceinline__ void z_face_part(Lattice19<R>& a, const Geometry& g, int i, int j)
 {
+    const int si = halo_side(i, g.l), sj = halo_side(j, g.m);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int kt = h == 0 ? 0 : g.n + 1, ks = h == 0 ? g.n : 1;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const int d = h == 0 ? kEnterZlo[q] : kEnterZhi[q];
+            const int is = XP && enters(si, cx_of(d)) ? wrap_to(si, g.l) : i;
+            const int js = enters(sj, cy_of(d)) ? wrap_to(sj, g.m) : j;
+            a.p[d][gidx(g, i, j, kt)] = a.p[d][gidx(g, is, js, ks)];
+        }
+    }
+}
+
+// bcond_periodic in one launch: threads [0, 10*x_face_rows) the x face,
+// the next (l+2)(n+2) the y face, the last (l+2)(m+2) the z face.  X = false
+// leaves the x face out and copies the x halo as it stands (bcond_inflow,
+// whose x faces are computed, not copied, and filled by inflow_x before this
+// launch; the y and z faces of SRC/bcond.inflow.F90 then copy them along).
+template <typename R, bool X>
+__global__ void periodic_faces(Lattice19<R> a, Geometry g)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (X) {
+        const long nxf = 10 * x_face_rows(g);
+        if (t < nxf) {
+            x_face_part<R, false>(a, g, t);
+            return;
+        }
+        t -= nxf;
+    }
     const int ni = g.l + 2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ni * (g.m + 2)) return;
-    const int i = t % ni, j = t / ni, n = g.n, n1 = g.n + 1;
-    BGK_A(4, i, j, 0) = BGK_A(4, i, j, n);
-    BGK_A(6, i, j, 0) = BGK_A(6, i, j, n);
-    BGK_A(7, i, j, 0) = BGK_A(7, i, j, n);
-    BGK_A(13, i, j, 0) = BGK_A(13, i, j, n);
-    BGK_A(18, i, j, 0) = BGK_A(18, i, j, n);
-    BGK_A(2, i, j, n1) = BGK_A(2, i, j, 1);
-    BGK_A(9, i, j, n1) = BGK_A(9, i, j, 1);
-    BGK_A(11, i, j, n1) = BGK_A(11, i, j, 1);
-    BGK_A(15, i, j, n1) = BGK_A(15, i, j, 1);
-    BGK_A(16, i, j, n1) = BGK_A(16, i, j, 1);
+    const long nyf = (long)ni * (g.n + 2), nzf = (long)ni * (g.m + 2);
+    if (t < nyf) {
+        y_face_part<R, false, X>(a, g, (int)(t % ni), (int)(t / ni));
+    } else if (t < nyf + nzf) {
+        const long u = t - nyf;
+        z_face_part<R, X>(a, g, (int)(u % ni), (int)(u / ni));
+    }
+}
+template <typename R, bool X>
+inline long periodic_faces_threads(const Geometry& g)
+{
+    return (X ? 10 * x_face_rows(g) : 0) + (long)(g.l + 2) * (g.n + 2) + (long)(g.l + 2) * (g.m + 2);
+}
+
+// bcond_channel in one launch (SRC/bcond.channel.F90: periodic x and y
+// faces, then the no-slip walls at z = 0, n+1 over the interior columns,
+// with the offload bounds 1..l x 1..m): the x face, the y face and the
+// walls (walls_z_cell, lid off), the faces skipping what a later face or a
+// wall overwrites.  The walls read only interior cells, and no face reads a
+// cell a wall writes (a face's source lies on the k of its target, and a
+// wall writes a population at a wall-plane cell only where that population
+// enters across the wall, which the faces' sources never are).
+template <typename R>
+__global__ void channel_faces(Lattice19<R> a, Geometry g)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nxf = 10 * x_face_rows(g);
+    if (t < nxf) {
+        x_face_part<R, true>(a, g, t);
+        return;
+    }
+    t -= nxf;
+    const int ni = g.l + 2;
+    const long nyf = (long)ni * (g.n + 2), nzf = (long)g.l * g.m;
+    if (t < nyf) {
+        y_face_part<R, true>(a, g, (int)(t % ni), (int)(t / ni));
+    } else if (t < nyf + nzf) {
+        const long u = t - nyf;
+        walls_z_cell(a, g, 1 + (int)(u % g.l), 1 + (int)(u / g.l), R(0), 0);
+    }
+}
+inline long channel_faces_threads(const Geometry& g)
+{
+    return 10 * x_face_rows(g) + (long)(g.l + 2) * (g.n + 2) + (long)g.l * g.m;
 }
 
 // inflow/outflow x faces over j=0..m+1, k=0..n+1
@@ -274,12 +343,20 @@ __global__ void inflow_x(Lattice19<R> a, Geometry g, R u_inflow, R rf, R qf, R p
 // Obstacle bounce-back (SRC/bcond.obs.F90:101-122): for every solid cell S
 // of the box, a_d(S) = a_opp(d)(S + c_d), which is what the fluid cell
 // X = S + c_d pulls as population d next (half-way bounce-back of its own
-// opposite population).  Only links whose neighbour S + c_d is fluid are
-// written, so every read comes from a fluid cell this kernel never writes:
-// the result is deterministic and fluid cells see exactly the reference's
-// values.  (The reference's own
-// GPU kernel and CPU loop also read solid neighbours they overwrite in the
-// same pass; those values only ever reach other solid cells.)
+// opposite population).
+//
+// The reference's CPU build runs this as one loop over the box, k slowest
+// and i fastest, in place.  When S + c_d = N is itself a solid cell of the
+// box that the loop visited before S, N's population opp(d) has already been
+// replaced by a_d(N + c_opp(d)) = a_d(S), so the loop leaves a_d(S) as it
+// was; in every other case it stores the old a_opp(d)(N).  Each thread
+// applies exactly that rule, which gives the serial loop's result in one
+// race-free pass: a thread reads a_opp(d)(N) only when N is not an earlier
+// solid cell, and N's thread then skips writing that population (S is
+// earlier for N), while a_d(S) is written only where N's thread does not
+// read it (N is fluid, outside the box, or later than S).  Solid cells
+// therefore end with the reference's values too, and so do the means over
+// all cells (SRC/diagno.F90:88-130).
 template <typename R>
 __global__ void obstacle_bb(Lattice19<R> a, Geometry g, const int* __restrict__ obs, int imin, int imax, int jmin,
                             int jmax, int kmin, int kmax)
@@ -290,18 +367,21 @@ __global__ void obstacle_bb(Lattice19<R> a, Geometry g, const int* __restrict__ 
     const int i = imin + (int)(t % ni);
     const int j = jmin + (int)((t / ni) % nj);
     const int k = kmin + (int)(t / ((long)ni * nj));
-    auto solid = [&](int ii, int jj, int kk) -> bool {
-        if (ii < 1 || ii > g.l || jj < 1 || jj > g.m || kk < 1 || kk > g.n) return false;
+    // a solid cell the loop over the box visits (the box lies in 1..l etc.)
+    auto visited_solid = [&](int ii, int jj, int kk) -> bool {
+        if (ii < imin || ii > imax || jj < jmin || jj > jmax || kk < kmin || kk > kmax) return false;
         return obs[(long)(ii - 1) + (long)g.l * (jj - 1) + (long)g.l * g.m * (kk - 1)] == 1;
     };
-    if (!solid(i, j, k)) return;
+    if (!visited_solid(i, j, k)) return;
     const long c = gidx(g, i, j, k);
 #pragma unroll
     for (int d = 0; d < 18; ++d) {
         // reference: a_d(i,j,k) = a_opp(d)(i + c_d)
-        const int ii = i + cx_of(d), jj = j + cy_of(d), kk = k + cz_of(d);
-        if (solid(ii, jj, kk)) continue;
-        a.p[d][c] = a.p[opp_of(d)][gidx(g, ii, jj, kk)];
+        const int dx = cx_of(d), dy = cy_of(d), dz = cz_of(d);
+        // the neighbour precedes (i,j,k) in the loop order (k, then j, then i)
+        const bool earlier = dz < 0 || (dz == 0 && (dy < 0 || (dy == 0 && dx < 0)));
+        if (earlier && visited_solid(i + dx, j + dy, k + dz)) continue;
+        a.p[d][c] = a.p[opp_of(d)][gidx(g, i + dx, j + dy, k + dz)];
     }
 }
 

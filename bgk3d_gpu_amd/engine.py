@@ -3,7 +3,8 @@
 `Engine(case, storage)` owns a lattice on the GPU laid out for gfx950 with
 the boundary conditions folded into the fused step.  Storage is 'f32' or
 'f64' (bit-identical to the reference's SINGLE / DOUBLE_P builds), or 'f16' /
-'bf16' (shifted populations in 16 bits, FP32 collision).
+'bf16' (shifted populations in 16 bits, FP32 collision).  `mixed=1` ('f16')
+and `mixed=2` ('f32') are the reference's MIXED1 and MIXED2 builds.
 """
 from __future__ import annotations
 
@@ -25,7 +26,7 @@ class BGKConfig(ctypes.Structure):
                 ("obs_ic", ctypes.c_int), ("obs_jc", ctypes.c_int), ("obs_kc", ctypes.c_int),
                 ("obs_radius", ctypes.c_int), ("u_inflow", ctypes.c_double), ("force_axis", ctypes.c_int),
                 ("device", ctypes.c_int), ("seed", ctypes.c_uint), ("les", ctypes.c_int),
-                ("cte_s", ctypes.c_double), ("wall_axis", ctypes.c_int)]
+                ("cte_s", ctypes.c_double), ("wall_axis", ctypes.c_int), ("mixed", ctypes.c_int)]
 
 
 def lib() -> ctypes.CDLL:
@@ -64,7 +65,10 @@ def _check(rc, what):
 
 
 def make_config(case: Case, storage: str = "f32", rounding: int = 0, obstacle: int | None = None,
-                centre=None, radius=None, u_inflow=None, device=0, seed=0, les=None, cte_s=None) -> BGKConfig:
+                centre=None, radius=None, u_inflow=None, device=0, seed=0, les=None, cte_s=None,
+                mixed: int = 0) -> BGKConfig:
+    """mixed: 1 the reference's MIXED1 form (storage 'f16', reference-order
+    FP32 collision), 2 its MIXED2 form (storage 'f32', FP64 collision)."""
     c = BGKConfig()
     lib().bgk_config_default(ctypes.byref(c), case.l, case.m, case.n)
     c.boundary = BOUNDARY[case.boundary]
@@ -85,6 +89,7 @@ def make_config(case: Case, storage: str = "f32", rounding: int = 0, obstacle: i
     c.les = 1 if (case.les if les is None else les) else 0
     c.cte_s = case.cte_s if cte_s is None else cte_s
     c.wall_axis = case.wall_axis
+    c.mixed = mixed
     return c
 
 

@@ -15,6 +15,11 @@ LIBDIR = Path(__file__).resolve().parent / "lib"
 _libs: dict[str, ctypes.CDLL] = {}
 
 
+# init-mode flag of bgk_init_host / bgk_init_gpu: unshifted storage (the
+# reference's -DNOSHIFT builds), BGK_INIT_UNSHIFTED in include/bgk3d_gpu.h
+INIT_UNSHIFTED = 8
+
+
 class BGKError(RuntimeError):
     pass
 

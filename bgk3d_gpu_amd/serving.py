@@ -76,10 +76,11 @@ class Request:
     rounding: int = 1         # stochastic rounding for 16-bit storage
     keep: bool = False
     tag: dict = field(default_factory=dict)
+    nu: float | None = None   # viscosity given directly (namelist svisc) instead of through re
 
     @property
     def svisc(self) -> float:
-        return self.u0 * self.n / self.re
+        return self.nu if self.nu is not None else self.u0 * self.n / self.re
 
     @property
     def tau(self) -> float:

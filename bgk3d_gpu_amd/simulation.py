@@ -45,6 +45,11 @@ class Simulation:
                 buf.upload(huge)
         self.omega, self.fgrad = c.hencol(prec)
         self.force = c.lid_force(prec)
+        if c.noshift and (c.les or c.boundary == "channel"):
+            raise ValueError("unshifted storage runs through col_mc_gpu: no channel force or LES form")
+        # storage convention: cte1 of the collide and the diagnostics, and of
+        # init and the inflow face (SRC/col_MC.F90:139-144, SRC/init.F90:71-75)
+        self.cte1 = 0.0 if c.noshift else 1.0
         self.obs_buf = None
         if c.obstacle:
             obs, self.box, self.radius, self.centre = sphere_obstacle(c.l, c.m, c.n)
@@ -62,7 +67,8 @@ class Simulation:
         host = np.full((19,) + c.shape, np.finfo(self.R).max, dtype=self.R)
         ptrs = (ctypes.c_void_p * 19)(*[host[d].ctypes.data for d in range(19)])
         self.lib.bgk_init_host.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [self.Rc]
-        rt.check(self.lib.bgk_init_host(ptrs, c.l, c.m, c.n, self.nx, self.ny, c.init, self.Rc(c.u00)),
+        mode = c.init | (rt.INIT_UNSHIFTED if c.noshift else 0)
+        rt.check(self.lib.bgk_init_host(ptrs, c.l, c.m, c.n, self.nx, self.ny, mode, self.Rc(c.u00)),
                  "bgk_init_host")
         return host
 
@@ -70,7 +76,7 @@ class Simulation:
         """Write the initial state straight into the device arrays with
         bgk_init_gpu (bit-identical to initial_state(); padding untouched)."""
         c = self.case
-        mode = c.init if mode is None else mode
+        mode = (c.init | (rt.INIT_UNSHIFTED if c.noshift else 0)) if mode is None else mode
         u00 = c.u00 if u00 is None else u00
         self.lib.bgk_init_gpu.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [self.Rc]
         rt.check(self.lib.bgk_init_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, mode, self.Rc(u00)), "bgk_init_gpu")
@@ -102,7 +108,7 @@ class Simulation:
         else:
             W = self.R
             L.bcond_inflow_gpu(*args, R(c.u_inflow), R(3.0), R(1.5), R(3.0), R(W(1.0 / 18.0)), R(W(1.0 / 36.0)),
-                               R(0.0))
+                               R(1.0 - self.cte1))
         if self.obs_buf is not None:
             L.bcond_obs_gpu(self.pa, self.obs_buf.ptr, c.l, c.m, c.n, self.nx, self.ny, *self.box, c.l, c.m)
 
@@ -115,9 +121,9 @@ class Simulation:
             L.col_mc_force_gpu(self.pa, self.pb, c.l, c.m, c.n, self.nx, self.ny, R(self.omega), R(self.fgrad),
                                c.force_axis)
         else:
-            L.col_mc_gpu(*self.pa, *list(self.pb)[:18], c.l, c.m, c.n, self.nx, self.ny, R(self.omega), R(0.0),
-                         R(1.0), R(W(1.0 / 3.0)), R(W(1.0 / 18.0)), R(W(1.0 / 36.0)), R(3.0), R(1.5), R(3.0),
-                         R(0.0), R(0.0), R(0.0))
+            L.col_mc_gpu(*self.pa, *list(self.pb)[:18], c.l, c.m, c.n, self.nx, self.ny, R(self.omega),
+                         R(1.0 - self.cte1), R(self.cte1), R(W(1.0 / 3.0)), R(W(1.0 / 18.0)), R(W(1.0 / 36.0)),
+                         R(3.0), R(1.5), R(3.0), R(0.0), R(0.0), R(0.0))
         for d in range(18):
             self.pa[d], self.pb[d] = self.pb[d], self.pa[d]
 
@@ -135,15 +141,15 @@ class Simulation:
         """{<u>, <v>, <w>, <rho>, <ke>} (SRC/diagno.F90)."""
         c = self.case
         out = np.zeros(5)
-        rt.check(self.lib.diagno_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, self.Rc(1.0), out.ctypes.data),
-                 "diagno_gpu")
+        rt.check(self.lib.diagno_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, self.Rc(self.cte1),
+                                     out.ctypes.data), "diagno_gpu")
         return out
 
     def varm(self):
         c = self.case
         u, w, v = (np.zeros(c.n) for _ in range(3))
         den = np.zeros(c.m)
-        rt.check(self.lib.varm_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, self.Rc(1.0), u.ctypes.data,
+        rt.check(self.lib.varm_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, self.Rc(self.cte1), u.ctypes.data,
                                    w.ctypes.data, v.ctypes.data, den.ctypes.data), "varm_gpu")
         return u, w, v, den
 
@@ -152,7 +158,7 @@ class Simulation:
         ic = c.l // 2 if ic is None else ic
         jc = c.m // 2 if jc is None else jc
         out = np.zeros((c.n, 4), dtype=self.R)
-        rt.check(self.lib.prof_k_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, ic, jc, self.Rc(1.0),
+        rt.check(self.lib.prof_k_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, ic, jc, self.Rc(self.cte1),
                                      out.ctypes.data), "prof_k_gpu")
         return out
 
@@ -161,7 +167,7 @@ class Simulation:
         jc = c.m // 2 if jc is None else jc
         kc = c.n // 2 if kc is None else kc
         out = np.zeros((c.l, 4), dtype=self.R)
-        rt.check(self.lib.prof_i_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, jc, kc, self.Rc(1.0),
+        rt.check(self.lib.prof_i_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, jc, kc, self.Rc(self.cte1),
                                      out.ctypes.data), "prof_i_gpu")
         return out
 
@@ -170,14 +176,14 @@ class Simulation:
         ic = c.l // 2 if ic is None else ic
         kc = c.n // 2 if kc is None else kc
         out = np.zeros((c.m, 4), dtype=self.R)
-        rt.check(self.lib.prof_j_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, ic, kc, self.Rc(1.0),
+        rt.check(self.lib.prof_j_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, ic, kc, self.Rc(self.cte1),
                                      out.ctypes.data), "prof_j_gpu")
         return out
 
     def probe(self, i0, j0, k0) -> np.ndarray:
         c = self.case
         out = np.zeros(4, dtype=self.R)
-        rt.check(self.lib.probe_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, i0, j0, k0, self.Rc(1.0),
+        rt.check(self.lib.probe_gpu(self.pa, c.l, c.m, c.n, self.nx, self.ny, i0, j0, k0, self.Rc(self.cte1),
                                     out.ctypes.data), "probe_gpu")
         return out
 

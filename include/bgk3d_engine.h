@@ -60,6 +60,10 @@ typedef struct bgk_config {
                              (default-real 0.1, SRC/input.F90:59) */
     int wall_axis;        /* channel walls: 3 at z = 1, n (the reference, SRC/bcond.channel.F90:
                              255-267), 2 at y = 1, m (z then periodic; force_axis 1 or 3) */
+    int mixed;            /* the reference's mixed-precision builds (SRC/precision.F90:30-43):
+                             0 none; 1 MIXED1: storage BGK_F16, the collision in FP32 in the
+                             reference's expression order (the plain F16 form uses a leaner
+                             FP32 arithmetic); 2 MIXED2: storage BGK_F32, collision in FP64 */
 } bgk_config;
 
 typedef struct bgk_handle bgk_handle;

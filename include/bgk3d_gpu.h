@@ -159,9 +159,14 @@ int probe_gpu(bgk_real_t** a, int l, int m, int n, int nx, int ny, int i0, int j
 /* Initial state f = f_eq into HOST arrays h[0..18] (all cells incl. halo),
  * shifted storage: mode 0 rest, mode 1 3-D Taylor-Green with amplitude u00,
  * mode 2 the same with the density perturbation kept (the reference's shifted
- * form drops it; no reference counterpart).
+ * form drops it; no reference counterpart).  Mode 0 or 1 plus
+ * BGK_INIT_UNSHIFTED stores f itself, as the reference's -DNOSHIFT builds
+ * (SRC/init.F90:71-75); such a lattice is stepped with col_mc_gpu's
+ * cte0 = 1, cte1 = 0, bcond_inflow_gpu's cte1 = 1 and read by the
+ * diagnostics with cte1 = 0.
  * Host-side like the reference's init, bit-identical to it.
  * Replaces SRC/init.F90:31-209 (PERIODIC + TGV3D branch for mode 1). */
+#define BGK_INIT_UNSHIFTED 8
 int bgk_init_host(bgk_real_t** h, int l, int m, int n, int nx, int ny, int mode, bgk_real_t u00);
 
 /* The same state written directly into the DEVICE arrays a[0..18] (all

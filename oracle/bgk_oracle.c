@@ -22,6 +22,17 @@
 #define COSf64 cos
 #define SQRTf64 sqrt
 
+/* Storage convention of the build: 0 shifted (f - w, the default), 1
+ * unshifted (the reference's -DNOSHIFT builds).  The reference sets cte1
+ * per routine: init and bcond_inflow use cte1 = 1 unshifted / 0 shifted
+ * (SRC/init.F90:71-75, SRC/bcond.inflow.F90:68-72); col_MC, diagno, varm,
+ * prof_i/j/k and probe use cte1 = 0 unshifted / 1 shifted, cte0 = 1 - cte1
+ * (SRC/col_MC.F90:139-144, SRC/diagno.F90:55-59, SRC/varm.F90:44-48,
+ * SRC/prof_i.F90:42-46, SRC/probe.F90:43-47).  Set with oracle_set_noshift. */
+static int g_noshift = 0;
+void oracle_set_noshift(int v) { g_noshift = v ? 1 : 0; }
+int oracle_get_noshift(void) { return g_noshift; }
+
 #define R float
 #define SFX f32
 #include "bgk_oracle_impl.h"

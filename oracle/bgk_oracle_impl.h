@@ -62,7 +62,7 @@ void FN(oracle_hencol_)(R svisc, R u0, int m, R *omega, R *fgrad)
 }
 
 /* ------------------------------------------------------------------------ */
-/* init: SRC/init.F90:31-209, shifted storage (cte1 = 0 there, :71-75).      */
+/* init: SRC/init.F90:31-209; cte1 = 0 shifted, 1 unshifted (:71-75).      */
 /* mode 0: rest flow (all populations 0).                                    */
 /* mode 1: 3-D Taylor-Green (PERIODIC + TGV3D, :114-148).                     */
 /* mode 2: mode 1 with the density perturbation kept (the reference's        */
@@ -75,7 +75,7 @@ void FN(oracle_init_)(R **a, int l, int m, int n, int nx, int ny, int mode, R u0
 {
     const size_t nxy = (size_t)nx * (size_t)ny;
     const R uno = (R)1, due = (R)2, tre = (R)3, qua = (R)4, zero = (R)0;
-    const R rf = (R)3, qf = (R)1.5, cte1 = zero;
+    const R rf = (R)3, qf = (R)1.5, cte1 = g_noshift ? uno : zero;
     const R p0 = FN(P0_), p1 = FN(P1_), p2 = FN(P2_);
     /* pi is a real(mykind) parameter whose value is a default-real (single)
      * literal (SRC/init.F90:63), so the DP build sees float-rounded pi. */
@@ -158,7 +158,7 @@ void FN(oracle_col_mc_)(R **a, R **b, int l, int m, int n, int nx, int ny, R ome
 {
     const size_t nxy = (size_t)nx * (size_t)ny;
     const R uno = (R)1, zero = (R)0;
-    const R cte1 = uno, cte0 = uno - cte1; /* shifted storage, :139-144 */
+    const R cte1 = g_noshift ? zero : uno, cte0 = uno - cte1; /* :139-144 */
     const R rf = (R)3, qf = (R)1.5, tre = (R)3;
     const R p0 = FN(P0_), p1 = FN(P1_), p2 = FN(P2_);
     R *const a01 = a[0], *const a02 = a[1], *const a03 = a[2], *const a04 = a[3], *const a05 = a[4];
@@ -427,6 +427,13 @@ static void FN(per_z_)(R **a, int l, int m, int n, int nx, int ny)
 }
 
 /* bcond_periodic: SRC/bcond.periodic.F90:67-266, faces x -> y -> z */
+/* the y and z faces alone (what bcond_inflow runs after its x faces) */
+void FN(oracle_faces_yz_)(R **a, int l, int m, int n, int nx, int ny)
+{
+    FN(per_y_)(a, l, m, n, nx, ny);
+    FN(per_z_)(a, l, m, n, nx, ny);
+}
+
 void FN(oracle_bcond_periodic_)(R **a, int l, int m, int n, int nx, int ny)
 {
     FN(per_x_)(a, l, m, n, nx, ny);
@@ -463,12 +470,12 @@ void FN(oracle_bcond_channel_)(R **a, int l, int m, int n, int nx, int ny)
 /* bcond_inflow: SRC/bcond.inflow.F90:91-328.  x=l+1 outflow equilibrium
  * (rho = 1, velocity = momentum at x = l, :131-162), x=0 inflow equilibrium
  * (rho = 1, u = (u_inflow,0,0), :165-181), then y and z periodic.
- * cte1 = 0 (shifted storage, :68-72).  `3.0` is a default-real literal. */
+ * cte1 = 0 shifted, 1 unshifted (:68-72).  `3.0` is a default-real literal. */
 void FN(oracle_bcond_inflow_)(R **a, int l, int m, int n, int nx, int ny, R u_inflow)
 {
     const size_t nxy = (size_t)nx * (size_t)ny;
     const int l1 = l + 1;
-    const R uno = (R)1, zero = (R)0, rf = (R)3, qf = (R)1.5, three = (R)3.0, cte1 = zero;
+    const R uno = (R)1, zero = (R)0, rf = (R)3, qf = (R)1.5, three = (R)3.0, cte1 = g_noshift ? uno : zero;
     const R p1 = FN(P1_), p2 = FN(P2_);
     for (int k = 0; k <= n + 1; ++k)
         for (int j = 0; j <= m + 1; ++j) {
@@ -557,7 +564,7 @@ void FN(oracle_bcond_obs_)(R **a, const int *obs, int l, int m, int n, int nx, i
 void FN(oracle_diagno_)(R **a, int l, int m, int n, int nx, int ny, double *out)
 {
     const size_t nxy = (size_t)nx * (size_t)ny;
-    const R uno = (R)1, cte1 = uno;
+    const R uno = (R)1, cte1 = g_noshift ? (R)0 : uno;
     double rtot = 0, xtot = 0, ytot = 0, ztot = 0, stot = 0;
     for (int k = 1; k <= n; ++k)
         for (int j = 1; j <= m; ++j)
@@ -598,7 +605,7 @@ void FN(oracle_diagno_)(R **a, int l, int m, int n, int nx, int ny, double *out)
 void FN(oracle_varm_)(R **a, int l, int m, int n, int nx, int ny, R *u_k, R *w_k, R *v_k, R *den_j)
 {
     const size_t nxy = (size_t)nx * (size_t)ny;
-    const R uno = (R)1, zero = (R)0, cte1 = uno;
+    const R uno = (R)1, zero = (R)0, cte1 = g_noshift ? zero : uno;
     for (int k = 0; k < n; ++k) u_k[k] = w_k[k] = v_k[k] = zero;
     for (int j = 0; j < m; ++j) den_j[j] = zero;
     for (int k = 1; k <= n; ++k)
@@ -639,7 +646,7 @@ void FN(oracle_varm_)(R **a, int l, int m, int n, int nx, int ny, R *u_k, R *w_k
  * out[4*(t-1) + {0,1,2,3}] = u, w (y-velocity), v (z-velocity), den. */
 static void FN(line_point_)(R **a, size_t s, R *o)
 {
-    const R uno = (R)1, cte1 = uno;
+    const R uno = (R)1, cte1 = g_noshift ? (R)0 : uno;
     const R den = +A(1)[s] + A(2)[s] + A(3)[s] + A(4)[s] + A(5)[s] + A(6)[s] + A(7)[s] + A(8)[s] + A(9)[s] +
                   A(10)[s] + A(11)[s] + A(12)[s] + A(13)[s] + A(14)[s] + A(15)[s] + A(16)[s] + A(17)[s] +
                   A(18)[s] + A(19)[s] + cte1;

@@ -54,6 +54,27 @@ RUNS_LES = [
     ("channel_24x16x20_les", "channel", 24, 16, 20, 200, 0.01, 0.02, 0, 100),
 ]
 
+# the reference's mixed-precision builds (SRC/precision.F90:30-43, Makefile
+# flags -DHALF_P -DMIXEDPRECISION and -DMIXEDPRECISION): m1 stores real*2
+# and computes in single precision, m2 stores single and computes in double.
+# MIXED1 sets the volume force to zero (SRC/hencol.F90:35-47), so its
+# channel would stay at rest: cavity and Taylor-Green only.
+RUNS_MIXED = [
+    ("ldc_24", "ldc", 24, 24, 24, 300, 0.064, 0.1, 0, 100, ("m1", "m2")),
+    ("tgv_32", "periodic", 32, 32, 32, 200, 0.01, 0.05, 1, 100, ("m1", "m2")),
+    ("ldc_64_re100", "ldc", 64, 64, 64, 2000, 0.064, 0.1, 0, 1000, ("m2",)),
+    ("channel_64x32x32", "channel", 64, 32, 32, 500, 0.1, 0.02048, 0, 250, ("m2",)),
+]
+
+# unshifted storage (-DNOSHIFT: f itself instead of f - w,
+# SRC/col_MC.F90:139-144, SRC/init.F90:71-75)
+RUNS_NOSHIFT = [
+    ("ldc_24", "ldc", 24, 24, 24, 300, 0.064, 0.1, 0, 100),
+    ("tgv_32", "periodic", 32, 32, 32, 200, 0.01, 0.05, 1, 100),
+    ("channel_64x32x32", "channel", 64, 32, 32, 500, 0.1, 0.02048, 0, 250),
+    ("inflow_sphere_100x48x48", "inflow", 100, 48, 48, 300, 0.02, 0.1, 0, 100),
+]
+
 # single-routine pins on seeded random states: (name, case, l, m, n, mode)
 UNIT = [
     ("bc_driven", "ldc", 13, 11, 9, 1),
@@ -84,7 +105,8 @@ def random_state(l, m, n, prec, seed):
 
 
 def main():
-    out = {"generator": "tests/golden/make_fixtures.py", "runs": {}, "unit": {}, "restart": {}, "original": {}}
+    out = {"generator": "tests/golden/make_fixtures.py", "runs": {}, "unit": {}, "restart": {}, "original": {},
+           "mixed": {}, "noshift": {}}
     for name, case, l, m, n, steps, svisc, u0, init_v, icheck in RUNS:
         for prec in ("sp", "dp"):
             if not O.ref_available(case, prec):
@@ -96,6 +118,9 @@ def main():
             out["runs"][f"{name}_{prec}"] = {
                 "case": case, "l": l, "m": m, "n": n, "steps": steps, "svisc": svisc, "u0": u0, "init": init_v,
                 "icheck": icheck, "prec": prec, "sha256_interior": digest(I),
+                # every interior cell, solid cells included (the serial
+                # loop order of SRC/bcond.obs.F90:101-122 fixes them)
+                "sha256_interior_all": digest(interior_bytes(A)),
                 "pop_sums": [float(x) for x in I.reshape(19, -1).astype(np.float64).sum(axis=1)],
                 "texts": {k: v for k, v in texts.items() if k not in ("obs", "files")},
                 "files": F.file_digests(texts["files"], l, m, n, prec),
@@ -116,6 +141,35 @@ def main():
                 "texts": {k: v for k, v in texts.items() if k not in ("obs", "files")},
             }
             print(name, prec, "done", flush=True)
+    # mixed-precision and unshifted builds: whole interior, text outputs
+    for name, case, l, m, n, steps, svisc, u0, init_v, icheck, precs in RUNS_MIXED:
+        for prec in precs:
+            if not O.ref_available(case, prec):
+                raise SystemExit(f"oracle/_ref/{case}_{prec} not built")
+            A, texts, _ = O.run_ref(case, l, m, n, steps, prec=prec, svisc=svisc, u0=u0, icheck=icheck,
+                                    init_v=init_v)
+            I = interior_bytes(A)
+            out["mixed"][f"{name}_{prec}"] = {
+                "case": case, "l": l, "m": m, "n": n, "steps": steps, "svisc": svisc, "u0": u0, "init": init_v,
+                "icheck": icheck, "prec": prec, "sha256_interior": digest(I),
+                "pop_sums": [float(x) for x in I.reshape(19, -1).astype(np.float64).sum(axis=1)],
+                "texts": {k: v for k, v in texts.items() if k not in ("obs", "files")},
+            }
+            print("mixed", name, prec, "done", flush=True)
+    for name, case, l, m, n, steps, svisc, u0, init_v, icheck in RUNS_NOSHIFT:
+        for prec in ("sp", "dp"):
+            if not O.ref_available(case, prec, variant="_noshift"):
+                raise SystemExit(f"oracle/_ref/{case}_{prec}_noshift not built")
+            A, texts, _ = O.run_ref(case, l, m, n, steps, prec=prec, svisc=svisc, u0=u0, icheck=icheck,
+                                    init_v=init_v, variant="_noshift")
+            I = interior_bytes(A)
+            out["noshift"][f"{name}_{prec}"] = {
+                "case": case, "l": l, "m": m, "n": n, "steps": steps, "svisc": svisc, "u0": u0, "init": init_v,
+                "icheck": icheck, "prec": prec, "sha256_interior": digest(I),
+                "pop_sums": [float(x) for x in I.reshape(19, -1).astype(np.float64).sum(axis=1)],
+                "texts": {k: v for k, v in texts.items() if k not in ("obs", "files")},
+            }
+            print("noshift", name, prec, "done", flush=True)
     # the ORIGINAL two-pass build (movef + col) ends on the fused build's state
     for name, case, l, m, n, steps, svisc, u0, init_v, icheck in RUNS[:1] + RUNS[2:4]:
         for prec in ("sp", "dp"):

@@ -46,8 +46,24 @@ def oracle_lib():
     return _lib
 
 
+class noshift:
+    """Context in which the oracle's routines use unshifted storage, the
+    reference's -DNOSHIFT builds (cte1 per routine as the reference sets it,
+    see oracle/bgk_oracle.c)."""
+
+    def __enter__(self):
+        oracle_lib().oracle_set_noshift(1)
+        return self
+
+    def __exit__(self, *exc):
+        oracle_lib().oracle_set_noshift(0)
+
+
 def np_dtype(prec: str):
-    return np.float32 if prec == "sp" else np.float64
+    """Storage dtype of a build: sp, dp, and the reference's mixed builds
+    m1 (real*2 storage, single compute) and m2 (single storage, double
+    compute), SRC/precision.F90:30-43."""
+    return {"sp": np.float32, "dp": np.float64, "m1": np.float16, "m2": np.float32}[prec]
 
 
 def ctype(prec: str):
@@ -304,6 +320,33 @@ def fortran_e(x: float, w: int = 14, d: int = 6) -> str:
 def diagno_line(itime: int, vals) -> str:
     """SRC/diagno.F90:145,159: (i8,5(e14.6,1x)) of itime, <u>, <v>, <w>, <rho>, <ke>."""
     return f"{itime:8d}" + "".join(fortran_e(v) + " " for v in vals)
+
+
+def parse_diagno(text: str):
+    """diagno.dat rows as [(itime, [5 floats]), ...]."""
+    rows = []
+    for line in text.splitlines():
+        if line.strip():
+            t, *v = line.split()
+            rows.append((int(t), [float(x) for x in v]))
+    return rows
+
+
+def assert_diagno_close(ours, ref_text, atol=1e-12):
+    """Our diagnostics [(itime, 5 values), ...] against a reference
+    diagno.dat: the same time steps, and every value within half a unit of
+    the file's last printed digit (e14.6: 6 significant digits) of the
+    printed value, except means that are zero up to rounding (|value| below
+    atol: the sum of antisymmetric velocities, which an FP64 sum in another
+    order leaves at a different 1e-17-size residue)."""
+    ref = parse_diagno(ref_text)
+    assert [t for t, _ in ours] == [t for t, _ in ref]
+    for (t, a), (_, b) in zip(ours, ref):
+        for q, (x, y) in enumerate(zip(np.asarray(a, dtype=float), b)):
+            if abs(y) < atol and abs(x) < atol:
+                continue
+            half_unit = 0.5 * 10.0 ** (np.floor(np.log10(abs(y))) + 1 - 6)
+            assert abs(x - y) <= half_unit * (1 + 1e-9), f"itime {t} column {q}: {x!r} vs printed {y!r}"
 
 
 def parse_blocks(text: str):

@@ -9,12 +9,21 @@
 * Admission: with a budget that holds two of the largest requests, no more
   than the budget is ever in use, requests start in arrival order and all finish.
 * The diagnostics the server reports equal bgk_diagno of the alone run.
+* Reference parity inside a mix: the golden cavity runs of the reference's
+  own CPU build (24^3 x 300 and 64^3 x 2000 steps, FP32 and FP64), served
+  among FP16/BF16 requests of other sizes, end on the reference's
+  populations bit for bit (sha256 of the interior) and report the
+  reference's last diagno.dat line (at print precision).
 """
+import hashlib
+import json
 import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
 
+import oracle_support as O
 from bgk3d_gpu_amd import runtime as rt
 from bgk3d_gpu_amd import serving as S
 from bgk3d_gpu_amd.engine import Engine
@@ -162,3 +171,37 @@ def test_shortest_first_policy(gpu):
     assert order == [ids[q] for q in np.argsort(work, kind="stable")]
     for q, r in enumerate(reqs):
         assert np.array_equal(pops[q], alone(r)[0], equal_nan=True)
+
+
+FIX = json.loads((Path(__file__).parent / "golden" / "fixtures.json").read_text())
+
+
+def test_served_requests_reproduce_reference_runs(gpu):
+    """SRC/bcond.driven.F90 + SRC/col_MC.F90 runs of the reference's CPU
+    build (tests/golden/fixtures.json), served in one mix with 16-bit
+    requests of other sizes, chunk 128 (so requests join and leave between
+    chunks that cut across the fixtures' step counts)."""
+    keys = ["ldc_24_sp", "ldc_24_dp", "ldc_64_re100_sp", "ldc_64_re100_dp"]
+    reqs = []
+    for q, key in enumerate(keys):
+        spec = FIX["runs"][key]
+        assert spec["case"] == "ldc" and spec["l"] == spec["m"] == spec["n"]
+        reqs.append(S.Request(spec["l"], 0.0, "f32" if spec["prec"] == "sp" else "f64", spec["steps"], seed=q,
+                              u0=spec["u0"], keep=True, nu=spec["svisc"]))
+    fillers = [S.Request(40, 400.0, "f16", 700, seed=90, rounding=1, keep=True),
+               S.Request(97, 1500.0, "bf16", 250, seed=91, rounding=1, keep=True),
+               S.Request(33, 200.0, "f16", 1100, seed=92, rounding=0, keep=True)]
+    banner(f"serving reference runs {keys} among {len(fillers)} 16-bit requests")
+    pops, res = serve(reqs + fillers, 128)
+    for q, key in enumerate(keys):
+        spec = FIX["runs"][key]
+        dt = np.float32 if spec["prec"] == "sp" else np.float64
+        P = pops[q].astype(dt)
+        assert np.array_equal(P.astype(np.float64), pops[q]), "storage values are exact in float64"
+        got = hashlib.sha256(np.ascontiguousarray(P).tobytes()).hexdigest()
+        assert got == spec["sha256_interior"], f"{key}: served populations differ from the reference's"
+        last = [x for x in spec["texts"]["diagno.dat"].splitlines() if x.strip()][-1]
+        O.assert_diagno_close([(spec["steps"], res[q]["diag"])], last)
+        assert not res[q]["unstable"]
+    for q in range(len(keys), len(reqs) + len(fillers)):
+        assert not res[q]["unstable"]

@@ -3,9 +3,10 @@
 Each case runs the reference time loop (boundaries -> col_MC -> swap) on
 the GPU via libbgk3d_gpu_{sp,dp}.so and on the CPU via the oracle (itself
 pinned bit-for-bit to the compiled reference in test_oracle_vs_reference).
-The bar is bit-identical populations on every fluid interior cell in both
-FP32 and FP64 -- stricter than the 1e-5 / 1e-12 relative tolerance of the
-goal, which the test also states explicitly.
+The bar is bit-identical arrays -- every interior cell, solid obstacle cells
+included, and the halo and padding -- in both FP32 and FP64, stricter than
+the 1e-5 / 1e-12 relative tolerance of the goal, which the test also states
+explicitly.
 """
 import sys
 
@@ -90,24 +91,28 @@ def test_time_loop_bit_exact(gpu, name, prec):
     try:
         sim.step(steps)
         G = sim.state()
+        dg = sim.diagno()
     finally:
         sim.close()
     A, obs = oracle_state(case, prec, steps)
     gi, oi = interior(G, case), interior(A, case)
-    if obs is not None:
-        fluid = obs == 0
-        gi, oi = gi[:, fluid], oi[:, fluid]
     assert np.isfinite(gi).all()
     scale = np.abs(oi).max()
     assert np.abs(gi.astype(np.float64) - oi).max() <= REL_TOL[prec] * scale
+    if obs is not None:
+        fluid = obs == 0
+        assert np.array_equal(np.ascontiguousarray(gi[:, fluid]).view(np.uint8),
+                              np.ascontiguousarray(oi[:, fluid]).view(np.uint8)), "fluid cells differ"
     assert np.array_equal(np.ascontiguousarray(gi).view(np.uint8), np.ascontiguousarray(oi).view(np.uint8)), \
         "not bit-identical to the oracle"
-    if obs is None:
-        # halo and padding too: the collide writes interior cells only and
-        # the boundary routines fill the same halo cells as the reference's
-        # (the channel walls with the offload bounds, as the oracle), so the
-        # arrays equal the oracle's everywhere
-        assert np.array_equal(G.view(np.uint8), A.view(np.uint8)), "halo or padding differs from the oracle"
+    # halo and padding too: the collide writes interior cells only and the
+    # boundary routines fill the same halo cells as the reference's (the
+    # channel walls with the offload bounds, as the oracle; the obstacle's
+    # solid cells with the values of the reference's serial loop), so the
+    # arrays equal the oracle's everywhere
+    assert np.array_equal(G.view(np.uint8), A.view(np.uint8)), "halo or padding differs from the oracle"
+    # the means run over every cell, solid ones included (SRC/diagno.F90:88-130)
+    np.testing.assert_allclose(dg, O.diagno(A, case.l, case.m, case.n, prec), rtol=1e-12, atol=1e-15)
 
 
 @pytest.mark.parametrize("prec", ["sp", "dp"])

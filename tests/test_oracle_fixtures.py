@@ -75,6 +75,9 @@ def test_oracle_reproduces_reference_run(key):
     sums = I.reshape(19, -1).astype(np.float64).sum(axis=1)
     np.testing.assert_allclose(sums, spec["pop_sums"], rtol=1e-6, atol=1e-9)
     assert digest(I) == spec["sha256_interior"], "interior populations differ from the reference"
+    if "sha256_interior_all" in spec:
+        # solid cells too: the oracle's bcond_obs runs the reference's serial loop order
+        assert digest(A[:, 1:-1, 1:-1, 1:-1]) == spec["sha256_interior_all"]
     # diagno.dat, line for line
     ref_lines = [x for x in spec["texts"]["diagno.dat"].splitlines() if x.strip()]
     ours = texts["diagno"]
@@ -140,6 +143,54 @@ def test_oracle_reproduces_reference_varm():
         for j, row in enumerate(rows):
             expect = f"{j + 1:5d}" + "".join(O.fortran_e(x) + " " for x in (u[j], w[j], den[j]))
             assert row.rstrip() == expect.rstrip(), (t, j, row, expect)
+
+
+NOSHIFT_KEYS = sorted(FIX["noshift"])
+
+
+@pytest.mark.parametrize("key", NOSHIFT_KEYS)
+def test_oracle_reproduces_reference_noshift_run(key):
+    """The reference's -DNOSHIFT builds (populations stored unshifted,
+    SRC/col_MC.F90:139-144, SRC/init.F90:71-75, SRC/bcond.inflow.F90:68-72,
+    SRC/diagno.F90:55-59): every interior cell and the text diagnostics."""
+    spec = FIX["noshift"][key]
+    with O.noshift():
+        A, obs, texts, _, _ = run_oracle(spec)
+    assert digest(A[:, 1:-1, 1:-1, 1:-1]) == spec["sha256_interior"]
+    for name, ours in (("diagno.dat", texts["diagno"]), ("probe.dat", texts["probe"])):
+        ref_lines = [x.rstrip() for x in spec["texts"][name].splitlines() if x.strip()]
+        assert ref_lines == [x.rstrip() for x in ours], name
+    blocks = O.parse_blocks(spec["texts"]["prof_k.dat"])
+    for t, rows in blocks.items():
+        pk = texts["prof_k"][t]
+        for q, row in enumerate(rows):
+            expect = "".join(O.fortran_e(v) + " " for v in (q + 0.5, *pk[q]))
+            assert row.rstrip() == expect.rstrip(), (t, q)
+
+
+def test_noshift_state_is_shifted_state_plus_weights():
+    """Unshifted and shifted runs of the same cavity hold the same physics:
+    after 50 steps the unshifted populations minus the weights equal the
+    shifted ones to rounding (the two differ only in where w_i is kept)."""
+    l = m = n = 12
+    res = {}
+    for flag in (False, True):
+        A = O.new_lattice(l, m, n, "dp")
+        B = O.new_lattice(l, m, n, "dp")
+        ctx = O.noshift() if flag else None
+        if ctx:
+            ctx.__enter__()
+        try:
+            O.init(A, l, m, n, "dp", mode=0, u00=0.1)
+            O.run("ldc", A, B, l, m, n, 50, prec="dp", svisc=0.05, u0=0.1)
+        finally:
+            if ctx:
+                ctx.__exit__(None, None, None)
+        res[flag] = A[:, 1:-1, 1:-1, 1:-1].copy()
+    w = np.array([1 / 36] * 4 + [1 / 18] * 2 + [1 / 36] + [1 / 18] + [1 / 36] * 5 + [1 / 18] * 2 + [1 / 36]
+                 + [1 / 18] + [1 / 36] + [1 / 3])
+    np.testing.assert_allclose(res[True] - w[:, None, None, None], res[False], atol=1e-14, rtol=0)
+    assert np.abs(res[False]).max() > 1e-4
 
 
 UNIT_KEYS = sorted(FIX["unit"])
