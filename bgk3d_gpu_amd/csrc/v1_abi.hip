@@ -40,6 +40,15 @@ hipStream_t& abi_stream()
 
 using namespace bgk;
 
+// Device scratch of diagno_gpu_noalloc: the partial sums of at most
+// kRedBlocks blocks and the folded totals, 5 doubles each (41 KB).  Each GPU
+// holds its own copy of this array, at its own address, from the library's
+// load until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (noalloc_scratch_ptr),
+// never through the array's name.
+constexpr size_t kNoAllocDoubles = (size_t)(kRedBlocks + 1) * 5;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
 namespace {
 
 Lattice19<real_t> lat_from(real_t** a)
@@ -174,7 +183,7 @@ Scratch& scratch_for_device()
     return s[dev & 63];
 }
 
-double* scratch(size_t doubles, double** host)
+double* cached_scratch(size_t doubles, double** host)
 {
     Scratch& s = scratch_for_device();
     if (s.cap < doubles) {
@@ -189,6 +198,29 @@ double* scratch(size_t doubles, double** host)
     }
     *host = s.host;
     return s.dev;
+}
+
+// The device address of `scratch` on the current device, looked up once per
+// device (first call there, with that device current) and kept; null if the
+// lookup fails.  Single host thread, as the reference's driver.
+double* noalloc_scratch_ptr()
+{
+    static double* ptr[64] = {};
+    static bool looked_up[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    if (!looked_up[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            std::fprintf(stderr, "bgk3d_gpu: diagno_gpu_noalloc: hipGetSymbolAddress failed: %s\n",
+                         hipGetErrorString(e));
+            return nullptr;
+        }
+        ptr[dev] = static_cast<double*>(p);
+        looked_up[dev] = true;
+    }
+    return ptr[dev];
 }
 
 int fetch(const double* dev, double* host, size_t n)
@@ -338,7 +370,7 @@ int diagno_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t cte1, dou
 {
     if (!shape_ok("diagno_gpu", l, m, n, nx, ny, a) || !out) return -1;
     double* host = nullptr;
-    double* dev = scratch((size_t)(kRedBlocks + 1) * 5, &host);
+    double* dev = cached_scratch((size_t)(kRedBlocks + 1) * 5, &host);
     if (!dev) return -2;
     const long rows = (long)m * n;
     const int nb = (int)(rows < kRedBlocks ? rows : kRedBlocks);
@@ -354,12 +386,39 @@ int diagno_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t cte1, dou
     return 0;
 }
 
+int diagno_gpu_noalloc(real_t** a, int l, int m, int n, int nx, int ny, real_t cte1, double* out)
+{
+    if (!shape_ok("diagno_gpu_noalloc", l, m, n, nx, ny, a) || !out) return -1;
+    const long rows = (long)m * n;
+    const int nb = (int)(rows < kRedBlocks ? rows : kRedBlocks);
+    // the partials of nb blocks after the 5 totals
+    if ((size_t)(nb + 1) * 5 > kNoAllocDoubles) {
+        std::fprintf(stderr, "bgk3d_gpu: diagno_gpu_noalloc: %d partials exceed the static scratch of %zu doubles\n",
+                     nb, kNoAllocDoubles);
+        return -1;
+    }
+    double* dev = noalloc_scratch_ptr();
+    if (!dev) return -2;
+    // every partial and total is written before it is read: nothing to clear
+    hipLaunchKernelGGL(diagno_partial<real_t>, dim3(nb), dim3(kRedThreads), 0, abi_stream(), lat_from(a),
+                       geom(l, m, n, nx, ny), cte1, dev + 5);
+    report_launch("diagno_partial");
+    hipLaunchKernelGGL(fold_partials<5>, dim3(1), dim3(kRedThreads), 0, abi_stream(), dev + 5, nb, dev);
+    report_launch("fold_partials");
+    double host[5];
+    BGK_HIP_CHECK(hipMemcpyAsync(host, dev, sizeof host, hipMemcpyDeviceToHost, abi_stream()));
+    BGK_HIP_CHECK(hipStreamSynchronize(abi_stream()));
+    const double fl = (double)(float)l, fm = (double)(float)m, fn = (double)(float)n;
+    for (int q = 0; q < 5; ++q) out[q] = (host[q] / fl) / fm / fn;
+    return 0;
+}
+
 int varm_gpu(real_t** a, int l, int m, int n, int nx, int ny, real_t cte1, double* u_k, double* w_k, double* v_k,
              double* den_j)
 {
     if (!shape_ok("varm_gpu", l, m, n, nx, ny, a) || !u_k || !w_k || !v_k || !den_j) return -1;
     double* host = nullptr;
-    double* dev = scratch((size_t)3 * n + m, &host);
+    double* dev = cached_scratch((size_t)3 * n + m, &host);
     if (!dev) return -2;
     const auto A = lat_from(a);
     const auto g = geom(l, m, n, nx, ny);
@@ -395,7 +454,7 @@ int draglift_gpu(real_t** a, const int* obs, int l, int m, int n, int nx, int ny
         return -1;
     }
     double* host = nullptr;
-    double* dev = scratch((size_t)(kRedBlocks + 1) * 3, &host);
+    double* dev = cached_scratch((size_t)(kRedBlocks + 1) * 3, &host);
     if (!dev) return -2;
     const long cells = (long)(ie - is + 1) * (je - js + 1) * (ke - ks + 1);
     const int nb = (int)((cells + kRedThreads - 1) / kRedThreads < kRedBlocks ? (cells + kRedThreads - 1) / kRedThreads
@@ -424,7 +483,7 @@ static int line_profile(real_t** a, int l, int m, int n, int nx, int ny, int axi
     if (c1 < 0 || c1 > lim1 + 1 || c2 < 0 || c2 > lim2 + 1) return -1;
     double* host = nullptr;
     const size_t bytes = (size_t)4 * count * sizeof(real_t);
-    double* dev = scratch((bytes + sizeof(double) - 1) / sizeof(double), &host);
+    double* dev = cached_scratch((bytes + sizeof(double) - 1) / sizeof(double), &host);
     if (!dev) return -2;
     real_t* d = reinterpret_cast<real_t*>(dev);
     hipLaunchKernelGGL(prof_line_kernel<real_t>, dim3(ceil_div(count, 64)), dim3(64), 0, abi_stream(), lat_from(a),

@@ -64,6 +64,7 @@ def v1(prec: str = "sp") -> ctypes.CDLL:
         lib.gpu_device_sync.argtypes = []
         lib.gpu_device_sync.restype = None
         lib.diagno_gpu.argtypes = [vp] + [ci] * 5 + [R, vp]
+        lib.diagno_gpu_noalloc.argtypes = [vp] + [ci] * 5 + [R, vp]
         lib.varm_gpu.argtypes = [vp] + [ci] * 5 + [R, vp, vp, vp, vp]
         lib.draglift_gpu.argtypes = [vp, vp] + [ci] * 9 + [R, vp]
         lib.prof_k_gpu.argtypes = [vp] + [ci] * 7 + [R, vp]

@@ -122,6 +122,20 @@ void col_mc_les_gpu(bgk_real_t** a, bgk_real_t** b, int l, int m, int n, int nx,
  * SRC/diagnostic.F90:91-94). */
 int diagno_gpu(bgk_real_t** a, int l, int m, int n, int nx, int ny, bgk_real_t cte1, double* out);
 
+/* diagno_gpu without any allocation, for callers that cannot allocate
+ * memory (code between graph captures, tight serving loops, hosts that
+ * forbid hipMalloc): no counterpart in the reference.  Same arguments,
+ * kernels and result as diagno_gpu; its partial sums live in a static
+ * device array of (1024 + 1) * 5 doubles per GPU (41 KB, held until the
+ * process ends) instead of the cached scratch and pinned buffer diagno_gpu
+ * allocates on first use.  The result comes back through a synchronous copy
+ * into `out`.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other host threads, other streams) race.  Returns 0, -1 for a
+ * bad argument or a reduction larger than the array (cannot happen for
+ * valid sizes: at most 1024 partials), -2 if the array's address cannot be
+ * obtained. */
+int diagno_gpu_noalloc(bgk_real_t** a, int l, int m, int n, int nx, int ny, bgk_real_t cte1, double* out);
+
 /* Mean profiles as varm forms them: x-, y- and z-velocity sums per z-plane k
  * (u_k, w_k, v_k, length n) and density sums per y-plane j (den_j, length
  * m), each times 1/l.  Replaces SRC/varm.F90:56-99. */
