@@ -12,9 +12,11 @@ roofline: dominant kernel (the fused collide), algorithmic bytes per launch =
           38 * 4 B * l*m*n (19 loads + 19 stores per interior cell), divided
           by the kernel's average duration measured with hipEvents around each
           launch in the timed region, against the 8 TB/s HBM peak.
-cpu_baseline: the reference's own Fortran CPU routines (oracle/_ref,
-          serial build) on a bounded sample of the same workload.
-forms:    FP64 / FP16 / BF16 storage and the reference-compatible ABI.
+cpu_baseline: the reference's own Fortran CPU routines (oracle/_ref): its
+          OpenMP loops on the host's cores, and its serial build, on a
+          bounded sample of the same workload.
+forms:    FP64 / FP16 / BF16 storage, the reference's mixed-precision forms,
+          the reference-compatible ABI, and the 1024x384x384 sphere.
 serving:  a reduced stream of lid-driven cavity requests (n in 32..192,
           Re in 100..3200, FP32 or FP16) served by the plain sequential loop
           and by the batching server: cell updates per second offline, and
@@ -64,25 +66,49 @@ def stream_bandwidth(lib, rt, nbytes=4 << 30, reps=10):
         dst.close()
 
 
+def host_threads():
+    """Cores this process may run on (the box's CPU share), at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def run_reference(build, case, steps, threads):
+    """The reference's routines (oracle/_ref/<build>) timed by their own loop
+    timer (SRC/finalize.F90:106-107): MLUPS, or None if the build is absent."""
+    exe = REPO / "oracle" / "_ref" / build / "ref_harness.x"
+    if not exe.exists():
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        out = subprocess.run([str(exe), d, str(case.l), str(case.m), str(case.n), repr(case.svisc),
+                              repr(case.u0), str(steps), str(steps + 1), "1", "0", "0", "3"],
+                             capture_output=True, text=True, timeout=600,
+                             env=dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close"))
+    for line in out.stdout.splitlines():
+        if line.startswith("# Mlups"):
+            return float(line.split()[-1])
+    raise RuntimeError(f"reference harness {build} failed: {out.stdout[-500:]} {out.stderr[-500:]}")
+
+
 def cpu_baseline(case, steps):
-    """Reference serial build timed on the host (bounded sample)."""
-    exe = REPO / "oracle" / "_ref" / "periodic_sp" / "ref_harness.x"
-    if exe.exists():
-        with tempfile.TemporaryDirectory() as d:
-            t0 = time.time()
-            out = subprocess.run([str(exe), d, str(case.l), str(case.m), str(case.n), repr(case.svisc),
-                                  repr(case.u0), str(steps), str(steps + 1), "1", "0", "0", "3"],
-                                 capture_output=True, text=True, timeout=600, env=dict(os.environ,
-                                                                                         OMP_NUM_THREADS="1"))
-            mlups = None
-            for line in out.stdout.splitlines():
-                if line.startswith("# Mlups"):
-                    mlups = float(line.split()[-1])
-            if mlups is None:
-                raise RuntimeError(f"reference harness failed: {out.stdout[-500:]} {out.stderr[-500:]}")
-            return {"value": mlups, "unit": "MLUPS", "cores": 1, "kind": "reference",
-                    "sample": f"reference Fortran CPU routines (amdflang -O3, serial), Taylor-Green "
-                              f"{case.l}x{case.m}x{case.n} FP32, {steps} steps; wall {time.time() - t0:.1f}s incl. init"}
+    """The reference's multicore CPU build on the host's cores, beside its
+    serial build (bounded sample of the same workload).  The multicore form
+    is the reference's OpenMP loops (-DOFFLOAD, private temporaries) run on
+    the host's threads: its `do concurrent` build mapped to OpenMP shares the
+    loop temporaries between threads and races (oracle/Makefile)."""
+    threads = host_threads()
+    t0 = time.time()
+    omp = run_reference("periodic_sp_omphost", case, steps, threads)
+    serial = run_reference("periodic_sp", case, steps, 1)
+    if omp is not None and serial is not None:
+        return {"value": omp, "unit": "MLUPS", "cores": threads, "kind": "reference",
+                "serial_value": serial, "serial_cores": 1, "host_cpus": os.cpu_count(),
+                "sample": f"reference Fortran CPU routines, Taylor-Green {case.l}x{case.m}x{case.n} FP32, "
+                          f"{steps} steps: OpenMP loops of its offload build run on {threads} host threads "
+                          f"(amdflang -O3 -fopenmp -DOFFLOAD, no offload target) and its serial build; "
+                          f"wall {time.time() - t0:.1f}s incl. init"}
     # fall back to the C oracle with host threads
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_support as O
@@ -99,6 +125,10 @@ def cpu_baseline(case, steps):
 
 
 def load_traffic(workload):
+    """HBM bytes per launch of the collide from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json, made by the
+    command in profiles/collide_fused_taylor_green_256_fp32.md); not measured
+    in this run."""
     p = REPO / "profiles" / "traffic.json"
     if p.exists():
         data = json.loads(p.read_text())
@@ -115,12 +145,12 @@ def bench_v1(case, prec, steps, warmup):
     return time_steps(case, prec, steps, warmup)
 
 
-def bench_engine(case, storage, steps, warmup, want_ke=False, rounding=None):
+def bench_engine(case, storage, steps, warmup, want_ke=False, rounding=None, **kw):
     from bgk3d_gpu_amd.engine import Engine
 
     if rounding is None:
-        rounding = 1 if storage in ("f16", "bf16") else 0
-    with Engine(case, storage, rounding=rounding) as eng:
+        rounding = 1 if storage in ("f16", "bf16") and not kw.get("mixed") else 0
+    with Engine(case, storage, rounding=rounding, **kw) as eng:
         ke0 = eng.diagno()[4] if want_ke else None
         eng.step(warmup)
         eng.sync()
@@ -130,6 +160,32 @@ def bench_engine(case, storage, steps, warmup, want_ke=False, rounding=None):
     if want_ke:
         info["ke0"] = ke0
     return total_ms, kern_ms, diag, info
+
+
+def bench_sphere(steps):
+    """Flow past a sphere, 1024 x 384 x 384 FP32 (Re = 300, R = 32 at
+    (256, 192, 192), inflow 0.05, solid mask): the handle API, and the
+    reference-compatible ABI driven the reference's way (bcond_inflow_gpu +
+    bcond_obs_gpu + col_mc_gpu per step, with the reference's shell mask at
+    its own centre and u_inflow, the case its build runs)."""
+    from bgk3d_gpu_amd.cases import Case
+
+    l, m, n, R, U = 1024, 384, 384, 32, 0.05
+    case = Case("inflow", l, m, n, svisc=U * 2 * R / 300.0, u0=U, obstacle=True)
+    t, k, d, info = bench_engine(case, "f32", steps, 3, obstacle=2, centre=(l // 4, m // 2, n // 2), radius=R,
+                                 u_inflow=U)
+    out = {"sphere_engine": {"mlups": round(case.cells * steps / (t * 1e-3) / 1e6, 1),
+                             "collide_us": round(k * 1e3, 1), "device_bytes": info["bytes"],
+                             "alg_GBs": round(38 * 4 * case.cells / (k * 1e-3) / 1e9, 1),
+                             "workload": "sphere 1024x384x384 FP32, solid R=32, Re=300"}}
+    log(f"sphere engine: {out['sphere_engine']}")
+    ref_case = Case("inflow", l, m, n, svisc=0.02, u0=0.1, obstacle=True)
+    t = bench_v1(ref_case, "sp", steps, 3)
+    out["sphere_v1_abi"] = {"mlups": round(ref_case.cells * steps / (t * 1e-3) / 1e6, 1),
+                            "ms_per_step": round(t / steps, 3),
+                            "workload": "sphere 1024x384x384 FP32, reference shell mask, reference layout"}
+    log(f"sphere v1: {out['sphere_v1_abi']}")
+    return out
 
 
 def bench_serving(count, steps, budget_gb=32.0, load=0.8, min_cells=4 << 20):
@@ -179,6 +235,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP64/FP16/BF16/v1 side measurements")
+    ap.add_argument("--no-sphere", action="store_true", help="skip the 1024x384x384 sphere forms")
     ap.add_argument("--serving-count", type=int, default=16, help="requests in the serving leg (0: skip)")
     ap.add_argument("--serving-steps", type=int, default=5000)
     args = ap.parse_args()
@@ -214,21 +271,28 @@ def main():
         fsteps = max(10, args.steps // 4)
         # 16-bit forms store with stochastic rounding (the accurate form);
         # *_rn is nearest-even rounding
-        for name, st, rnd, sb in (("f64", "f64", 0, 8), ("f16", "f16", 1, 2), ("f16_rn", "f16", 0, 2),
-                                  ("bf16", "bf16", 1, 2), ("bf16_rn", "bf16", 0, 2)):
-            if st == storage:
+        # mixed1 / mixed2: the reference's MIXED1 (FP16 storage, reference-order
+        # FP32 collide, nearest) and MIXED2 (FP32 storage, FP64 collide) builds
+        for name, st, rnd, sb, mixed in (("f64", "f64", 0, 8, 0), ("f16", "f16", 1, 2, 0),
+                                         ("f16_rn", "f16", 0, 2, 0), ("bf16", "bf16", 1, 2, 0),
+                                         ("bf16_rn", "bf16", 0, 2, 0), ("f16_mixed1", "f16", 0, 2, 1),
+                                         ("f32_mixed2", "f32", 0, 4, 2)):
+            if st == storage and not mixed:
                 continue
-            t, k, d, _ = bench_engine(case, st, fsteps, 5, rounding=rnd)
+            t, k, d, _ = bench_engine(case, st, fsteps, 5, rounding=rnd, mixed=mixed)
             forms[name] = {"mlups": round(case.cells * fsteps / (t * 1e-3) / 1e6, 1),
                            "collide_us": round(k * 1e3, 2),
                            "alg_GBs": round(38 * sb * case.cells / (k * 1e-3) / 1e9, 1)}
             log(f"{name}: {forms[name]}")
         t = bench_v1(case, args.prec, fsteps, 5)
         forms["v1_abi_" + storage] = {"mlups": round(case.cells * fsteps / (t * 1e-3) / 1e6, 1),
+                                      "ms_per_step": round(t / fsteps, 4),
                                       "note": "reference layout (nx=l+2), halo kernels per step"}
         log(f"v1: {forms['v1_abi_' + storage]}")
         if "f16" in forms:
             forms["f16_over_" + storage] = round(forms["f16"]["mlups"] / mlups, 3)
+        if not args.no_sphere:
+            forms.update(bench_sphere(max(5, fsteps // 2)))
     stream = None
     if not args.no_stream:
         stream = stream_bandwidth(rt.v1("sp"), rt)
@@ -243,6 +307,7 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(case, args.cpu_steps)
         log(f"cpu baseline {cpu['value']:.2f} MLUPS ({cpu['kind']}, {cpu['cores']} cores)")
+    traffic = load_traffic(workload)
     out = {
         "metric": METRIC,
         "value": round(mlups, 1),
@@ -260,7 +325,10 @@ def main():
                    "path": "handle API: aligned layout, periodic boundary folded into the fused step",
                    "layout_nx": info["nx"], "device_bytes": info["bytes"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(workload),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE passes of this "
+                                       "kernel (profiles/collide_fused_taylor_green_256_fp32.md), not this run",
+                     "traffic_over_alg": round(traffic / alg_bytes, 4) if traffic else None,
                      "kernel": "collide_fused", "kernel_avg_us": round(kern_avg_ms * 1e3, 2),
                      "alg_bytes_per_launch": alg_bytes,
                      "measured_copy_GBs": round(stream, 1) if stream else None,

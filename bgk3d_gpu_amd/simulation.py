@@ -22,7 +22,11 @@ P_CONSTS = {"sp": np.float32, "dp": np.float64}
 
 
 class Simulation:
-    def __init__(self, case: Case, prec: str = "sp", poison: bool = True):
+    """poison: fill every array with huge() first (the reference's alloca);
+    device_init: write f = f_eq with bgk_init_gpu instead of computing it on
+    the host and uploading it (same bits; for lattices of many GB)."""
+
+    def __init__(self, case: Case, prec: str = "sp", poison: bool = True, device_init: bool = False):
         if prec not in ("sp", "dp"):
             raise ValueError("prec must be 'sp' or 'dp'")
         self.case, self.prec = case, prec
@@ -57,7 +61,10 @@ class Simulation:
             self.obs_buf = rt.DeviceBuffer(obs.nbytes, self.lib)
             self.obs_buf.upload(obs)
         self.itime = 0
-        self.set_state(self.initial_state())
+        if device_init:
+            self.init_device()
+        else:
+            self.set_state(self.initial_state())
 
     # ---- state -----------------------------------------------------------
     def initial_state(self) -> np.ndarray:

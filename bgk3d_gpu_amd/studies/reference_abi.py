@@ -27,7 +27,7 @@ from ..simulation import Simulation
 
 
 def time_steps(case: Case, prec: str, steps: int, warmup: int) -> float:
-    sim = Simulation(case, prec, poison=False)
+    sim = Simulation(case, prec, poison=False, device_init=True)
     try:
         sim.step(warmup)
         sim.sync()

@@ -276,3 +276,22 @@ def test_original_two_pass_build_equals_fused(key):
                         svisc=spec["svisc"], u0=spec["u0"], icheck=spec["icheck"], init_v=spec["init"],
                         variant="_orig")
     assert digest(np.ascontiguousarray(A[:, 1:-1, 1:-1, 1:-1])) == spec["sha256_interior"]
+
+
+@pytest.mark.parametrize("case,key", [("periodic", "tgv_32_sp"), ("ldc", "ldc_24_sp")])
+def test_threaded_reference_build_reproduces_serial_run(case, key):
+    """The multicore CPU baseline: the reference's OpenMP loops (-DOFFLOAD
+    branches, with their private lists) built without an offload target and
+    run on 4 host threads end on the serial build's fixture state bit for
+    bit.  (Its `do concurrent` build mapped to OpenMP, -fdo-concurrent-to-
+    openmp=host, shares the loop temporaries of SRC/col_MC.F90:272-519
+    between threads and does not: that is why bench.py times this one.)"""
+    spec = FIX["runs"][key]
+    if not O.ref_available(case, "sp", variant="_omphost"):
+        pytest.skip("threaded reference build not available")
+    A, texts, _ = O.run_ref(case, spec["l"], spec["m"], spec["n"], spec["steps"], prec="sp", svisc=spec["svisc"],
+                            u0=spec["u0"], icheck=spec["icheck"], init_v=spec["init"], threads=4,
+                            variant="_omphost")
+    assert digest(A[:, 1:-1, 1:-1, 1:-1]) == spec["sha256_interior"]
+    ref_lines = [x.rstrip() for x in spec["texts"]["diagno.dat"].splitlines() if x.strip()]
+    assert [x.rstrip() for x in texts["diagno.dat"].splitlines() if x.strip()] == ref_lines
