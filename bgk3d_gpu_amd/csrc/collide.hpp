@@ -64,7 +64,19 @@ struct FusedArgs {
     // Distance in elements between populations d and d+1 of one lattice
     // (library-owned layout; 0 for the reference ABI, which never uses it).
     long pstride;
+    // Non-finite flag (serving): set to 1 when a cell's new rest population
+    // is not finite -- a non-finite density or velocity anywhere in the
+    // cell's inputs makes it so.  Null: no check.
+    uint32_t* bad;
 };
+
+// Raise the non-finite flag of p (if any) when the rest population v is not
+// finite.  Only lanes that see one store, so a healthy step stores nothing.
+template <typename S>
+__device__ __forceinline__ void flag_nonfinite(const FusedArgs<S>& p, typename Storage<S>::compute v)
+{
+    if (p.bad && !__builtin_isfinite(v)) *gmem(p.bad) = 1u;
+}
 
 // Base of population d of the lattice read (A) or written (B) this step.
 // CMP (compact): population d < 18 of a lattice at a[0] + d*pstride or
@@ -298,6 +310,84 @@ inline RowMap row_map_capped(int span, int maxw)
     return r;
 }
 
+// Zeros into population d..18 of one cell that no boundary reads (row
+// padding, a halo cell of a folded boundary), so that its line is written
+// whole.
+template <typename S, bool CMP>
+__device__ __forceinline__ void store_zero_cell(const FusedArgs<S>& p, long row, int i)
+{
+    typedef typename Storage<S>::compute C;
+    constexpr long ES = (long)sizeof(S);
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        S* dst;
+        if constexpr (sizeof(S) >= 4)
+            dst = (d < 18 ? p.b[d] : p.a[18]) + (row + i);
+        else
+            dst = reinterpret_cast<S*>(reinterpret_cast<char*>(pop_b<CMP>(p, d)) + row * ES +
+                                       (uint64_t)((uint32_t)i * (uint32_t)ES));
+        store_nt(dst, Rounder<S, 0>::cvt(C(0), 0u));
+    }
+}
+
+// Collision and stores of one cell whose pulled populations are x.  `cell`
+// keys the stochastic-rounding stream: the cell's index in the reference's
+// (0:l+1, 0:m+1, 0:n+1) layout, so the stream does not depend on the
+// wave mapping or the row padding.
+template <typename S, int FMODE, int SR, bool CMP>
+__device__ __forceinline__ void collide_store(const FusedArgs<S>& p, typename Storage<S>::compute (&x)[19], int i,
+                                              long row, uint32_t seed, uint32_t cell)
+{
+    typedef typename Storage<S>::compute C;
+    constexpr long ES = (long)sizeof(S);
+    const uint32_t vi = (uint32_t)i * (uint32_t)ES;
+    // FMODE bits 0-1: body force (0 uniform, 1/2/3 fgrad*rho along x/y/z);
+    // bit 2: Smagorinsky LES relaxation
+    constexpr bool LES = (FMODE & 4) != 0;
+    C fx, fy, fz;
+    force_terms<C, (FMODE & 3)>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
+    if constexpr (std::is_same<C, double>::value && !LES) {
+        // FP64 arithmetic: reference-order, each value stored as soon as it
+        // is formed (77 vector registers instead of 86: 6 waves per SIMD
+        // without spilling).  The FP32 form keeps the array collide: with
+        // its stores spread through the arithmetic, rows that end in a
+        // partly interior line (n = 134 ... 182) ran 18-24 % slower.
+        bgk_collide_stream(x, p.omega, fx, fy, fz, p.cst, [&](auto dc, C v) {
+            constexpr int d = decltype(dc)::value;
+            store_nt((d < 18 ? p.b[d] : p.a[18]) + (row + i), Rounder<S, 0>::cvt(v, 0u));
+            if constexpr (d == 18) flag_nonfinite(p, v);
+        });
+        return;
+    }
+    C o[19];
+    if constexpr (Storage<S>::lean)
+        bgk_collide_lean<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
+    else
+        bgk_collide_ref<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
+
+    // stochastic rounding: one hash per cell and step, then a Weyl sequence
+    // over the cell's stores (deterministic: independent of scheduling)
+    uint32_t r = 0;
+    if (SR) r = hash32(seed ^ cell);
+#pragma unroll
+    for (int d = 0; d < 19; ++d) {
+        S* dst;
+        if constexpr (sizeof(S) >= 4)
+            dst = (d < 18 ? p.b[d] : p.a[18]) + (row + i);
+        else
+            dst = reinterpret_cast<S*>(reinterpret_cast<char*>(pop_b<CMP>(p, d)) + row * ES + (uint64_t)vi);
+        store_nt(dst, Rounder<S, SR>::cvt(o[d], SR ? sr_word(r, d) : 0u));
+    }
+    flag_nonfinite(p, o[18]);
+}
+
+// Index of cell (i, j, k) in the reference's (0:l+1, 0:m+1, 0:n+1) layout.
+template <typename S>
+__device__ __forceinline__ uint32_t ref_cell(const FusedArgs<S>& p, int i, int j, int k)
+{
+    return (uint32_t)i + (uint32_t)(p.l + 2) * ((uint32_t)j + (uint32_t)(p.m + 2) * (uint32_t)k);
+}
+
 // The work of one block (bx, by, bz) of the launch geometry `rm` describes:
 // pull, collide, store.  `seed` is the step's stochastic-rounding seed.
 template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
@@ -305,7 +395,6 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
                                             uint32_t seed)
 {
     typedef typename Storage<S>::compute C;
-    constexpr long ES = (long)sizeof(S);
     constexpr bool CMP = kCompact<S, FMODE>;
     // the 16-bit forms declare the wave index scalar, so that the row offset
     // and the boundary decisions stay in scalar registers
@@ -331,25 +420,13 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     const int i = 1 - shift + seg * 64 + lane;
     if (i < 1) return;
     const long row = (long)p.nx * j + p.nxy * (long)k;  // scalar
-    const uint32_t vi = (uint32_t)i * (uint32_t)ES;
     if (i > p.l) {
         // Whole-line stores: when the row's last 128-byte line is only partly
         // interior, the lanes past i = l fill it (row padding and the next
         // row's i = 0 halo cell, which a folded x boundary never reads), so
         // HBM sees full-line writes instead of partial ones.  Only with a
         // layout whose rows start on a line (lend set by the engine).
-        if (i <= p.lend) {
-#pragma unroll
-            for (int d = 0; d < 19; ++d) {
-                S* dst;
-                if constexpr (sizeof(S) >= 4)
-                    dst = (d < 18 ? p.b[d] : p.a[18]) + (row + i);
-                else
-                    dst = reinterpret_cast<S*>(reinterpret_cast<char*>(pop_b<CMP>(p, d)) + row * ES +
-                                               (uint64_t)vi);
-                store_nt(dst, Rounder<S, 0>::cvt(C(0), 0u));
-            }
-        }
+        if (i <= p.lend) store_zero_cell<S, CMP>(p, row, i);
         return;
     }
 
@@ -359,43 +436,40 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
         pull_all<S, BX, BY, BZ, OBST, true, CMP>(p, i, j, k, row, x);
     else
         pull_all<S, BX, BY, BZ, OBST, false, CMP>(p, i, j, k, row, x);
+    collide_store<S, FMODE, SR, CMP>(p, x, i, row, seed, ref_cell(p, i, j, k));
+}
 
-    // FMODE bits 0-1: body force (0 uniform, 1/2/3 fgrad*rho along x/y/z);
-    // bit 2: Smagorinsky LES relaxation
-    constexpr bool LES = (FMODE & 4) != 0;
-    C fx, fy, fz;
-    force_terms<C, (FMODE & 3)>(density_ref(x, p.cst.cte1), p.fgrad, p.fx, p.fy, p.fz, fx, fy, fz);
-    if constexpr (std::is_same<C, double>::value && !LES) {
-        // FP64 arithmetic: reference-order, each value stored as soon as it
-        // is formed (77 vector registers instead of 86: 6 waves per SIMD
-        // without spilling).  The FP32 form keeps the array collide: with
-        // its stores spread through the arithmetic, rows that end in a
-        // partly interior line (n = 134 ... 182) ran 18-24 % slower.
-        bgk_collide_stream(x, p.omega, fx, fy, fz, p.cst, [&](auto dc, C v) {
-            constexpr int d = decltype(dc)::value;
-            store_nt((d < 18 ? p.b[d] : p.a[18]) + (row + i), Rounder<S, 0>::cvt(v, 0u));
-        });
+// The compact layout of the handle API: rows of l + 2 cells without padding
+// (nx = l + 2), each population plane padded to whole 128-byte lines and
+// placed so that cell (0, 1, k) starts a line.  The waves of plane k cover
+// its rows j = 1..m as one range of positions r = i + nx (j - 1), 64
+// consecutive cells per wave, line-aligned, up to the line end after
+// (l+1, m) (p.lend positions): lanes on interior cells pull, collide and
+// store; lanes on the x halo cells and on the cells of row m+1 store zeros,
+// so every line is written whole.  Only with folded boundaries, which never
+// read a halo cell.  The padded layout spends whole lines on a row's last
+// few cells (a 129-cell FP32 row spans five lines for 4.03 lines of data);
+// this one wastes 2 cells per row, with rows that start anywhere in a line
+// (y-shifted pulls then span three lines per wave, the third shared with the
+// neighbouring wave through L2).  bz = k - 1, bx = position block of 256.
+template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
+__device__ __forceinline__ void compact_cells(const FusedArgs<S>& p, int bx, int bz, uint32_t seed)
+{
+    typedef typename Storage<S>::compute C;
+    constexpr bool CMP = kCompact<S, FMODE>;
+    const int r = bx * 256 + (int)threadIdx.x;
+    if (r >= p.lend) return;
+    const int k = 1 + bz;
+    const int j = 1 + r / p.nx;
+    const int i = r - (j - 1) * p.nx;
+    const long row = (long)p.nx * j + p.nxy * (long)k;
+    if (j > p.m || i == 0 || i > p.l) {
+        store_zero_cell<S, CMP>(p, row, i);
         return;
     }
-    C o[19];
-    if constexpr (Storage<S>::lean)
-        bgk_collide_lean<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
-    else
-        bgk_collide_ref<C, LES>(x, p.omega, fx, fy, fz, p.cst, o, p.cte_s);
-
-    // stochastic rounding: one hash per cell and step, then a Weyl sequence
-    // over the cell's stores (deterministic: independent of scheduling)
-    uint32_t r = 0;
-    if (SR) r = hash32(seed ^ (uint32_t)(row + i));
-#pragma unroll
-    for (int d = 0; d < 19; ++d) {
-        S* dst;
-        if constexpr (sizeof(S) >= 4)
-            dst = (d < 18 ? p.b[d] : p.a[18]) + (row + i);
-        else
-            dst = reinterpret_cast<S*>(reinterpret_cast<char*>(pop_b<CMP>(p, d)) + row * ES + (uint64_t)vi);
-        store_nt(dst, Rounder<S, SR>::cvt(o[d], SR ? sr_word(r, d) : 0u));
-    }
+    C x[19];
+    pull_all<S, BX, BY, BZ, OBST, true, CMP>(p, i, j, k, row, x);
+    collide_store<S, FMODE, SR, CMP>(p, x, i, row, seed, ref_cell(p, i, j, k));
 }
 
 // Waves per SIMD the compiler is asked to fit (1: no request).  FP64
@@ -414,6 +488,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FusedWaves<
     FusedArgs<S> p, RowMap rm)
 {
     fused_rows<S, BX, BY, BZ, OBST, FMODE, SR>(p, rm, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, p.seed);
+}
+
+template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FusedWaves<S, FMODE>::value))) void collide_compact(
+    FusedArgs<S> p)
+{
+    compact_cells<S, BX, BY, BZ, OBST, FMODE, SR>(p, (int)blockIdx.x, (int)blockIdx.y, p.seed);
 }
 
 // Reference-layout form (the caller's arrays, every boundary already in the

@@ -221,6 +221,13 @@ template <typename S, int BX, int BY, int BZ, bool OB, int FM, int SR>
 void launch_fused(bgk_handle& h)
 {
     const FusedArgs<S> p = fused_args<S>(h, h.cur);
+    if constexpr (BX != AX_HALO) {
+        if (h.compact) {
+            hipLaunchKernelGGL((collide_compact<S, BX, BY, BZ, OB, FM, SR>),
+                               dim3((unsigned)ceil_div(p.lend, 256), (unsigned)p.n), dim3(256), 0, h.stream, p);
+            return;
+        }
+    }
     const RowMap rm = row_map(p.l);
     hipLaunchKernelGGL((collide_fused<S, BX, BY, BZ, OB, FM, SR>), fused_grid(rm, p.m, p.n), fused_block(rm), 0,
                        h.stream, p, rm);
@@ -358,15 +365,30 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
                             "the seam",
                             ctr[ax], R + 4, size[ax], "xyz"[ax]);
     }
+    if (c.layout < 0 || c.layout > 2) return fail(BGK_EINVAL, "layout %d", c.layout);
+    if (c.layout == 2 && c.boundary == BGK_BC_INFLOW)
+        return fail(BGK_EINVAL, "the compact layout needs folded x boundaries (not inflow/outflow)");
     h.cfg = c;
     h.esize = storage_bytes(c.storage);
     h.A = (int)(128 / h.esize);
-    h.nx = (int)round_up(c.l + 2, h.A);
     h.ny = c.m + 2;
     h.nz = c.n + 2;
-    h.plane = (long)h.nx * h.ny;
+    // Rows padded to whole lines, unless the last line of a row would hold
+    // more than kCompactWaste of a row in unused cells: then compact rows.
+    const double waste = (double)(round_up(c.l, h.A) - c.l) / (double)c.l;
+    h.compact = c.boundary != BGK_BC_INFLOW && (c.layout == 2 || (c.layout == 0 && waste > kCompactWaste));
+    if (h.compact) {
+        // cell (0, 1, k) of every population plane on a line start
+        h.nx = c.l + 2;
+        h.plane = round_up((long)h.nx * h.ny, h.A);
+        h.base = (h.A - h.nx % h.A) % h.A;
+    } else {
+        // cell (1, j, k) on a line start
+        h.nx = (int)round_up(c.l + 2, h.A);
+        h.plane = (long)h.nx * h.ny;
+        h.base = h.A - 1;
+    }
     h.nxy = 19 * h.plane;
-    h.base = h.A - 1;
     h.lstride = round_up(h.A + h.nxy * h.nz, h.A);
     // derived parameters in the collision kind (SRC/hencol.F90:31, :52;
     // SRC/bcond.driven.F90:61; SRC/build.bcond.F90:29-35)
@@ -412,6 +434,7 @@ void bgk_config_default(bgk_config* c, int l, int m, int n)
     c->cte_s = -1.0;
     c->wall_axis = 3;
     c->mixed = 0;
+    c->layout = 0;
 }
 
 const char* bgk_last_error(void) { return last_error(); }

@@ -16,12 +16,22 @@
 // index, request seed and the request's own step count).
 //
 // Steps run in chunks of at most `chunk_steps`, shortened so that no request
-// overshoots its step count; requests join and leave only between chunks.
+// overshoots its step count and, with a chunk-time target, so that a chunk
+// takes about that long (requests join and leave only between chunks, so
+// the target bounds how long an arrival waits to start).  The storage
+// groups of a chunk run on two streams, step by step, so their launches
+// overlap.  Every cell's new rest population is checked for a non-finite
+// value inside the step kernel; a request whose solution became non-finite
+// is retired as unstable at the end of the chunk instead of running to its
+// last step.
 // Policies: every admitted request advances in every chunk (processor
 // sharing, arrival-order admission), or only the admitted requests with the
 // least work left, as many as keep a given number of cells per step in
 // flight (shortest remaining work first, which cuts waiting for the small
-// requests of a mix while the GPU stays saturated).
+// requests of a mix while the GPU stays saturated); a request passed over
+// for kMaxSkips chunks in a row is advanced (and a queued one admitted)
+// ahead of the rest, so a stream of small requests cannot starve a large
+// one.
 #include "engine_internal.hpp"
 
 #include <algorithm>
@@ -49,12 +59,17 @@ constexpr size_t kArenaAlign = 64 << 10;
 // stay cached from one launch to the next; every request still advances all
 // the chunk's steps in order, so results do not depend on the grouping.
 constexpr size_t kSubBatchBytes = size_t(3) << 30;
+// Requests admitted at once: each holds one non-finite flag word.
+constexpr int kMaxFlags = 4096;
+// Shortest-first aging: chunks a request may be passed over in a row.
+constexpr int kMaxSkips = 8;
 
 template <typename S>
 struct BatchSlot {
     FusedArgs<S> args[2];  // indexed by the lattice the step reads
     RowMap rm;
-    int gy;                // row blocks along j
+    int gy;                // row blocks along j (padded rows)
+    int nbx;               // position blocks per plane (compact rows)
     int par0;              // lattice read at chunk step 0
     uint32_t cfg_seed;
     long long steps0;      // steps completed before the chunk
@@ -77,6 +92,22 @@ collide_batched(const BatchSlot<S>* __restrict__ slots, const uint32_t* __restri
     const int bz = (int)(rest / (uint32_t)sl.gy);
     const uint32_t seed = SR ? step_seed(sl.cfg_seed, sl.steps0 + t) : 0u;
     fused_rows<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>(sl.args[(sl.par0 + t) & 1], rm, bx, by, bz, seed);
+}
+
+// The same for requests in the compact layout (collide_compact's mapping,
+// 256-thread blocks): a request's blocks are nbx position blocks per plane.
+template <typename S, int SR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FusedWaves<S, 0>::value))) void
+collide_batched_compact(const BatchSlot<S>* __restrict__ slots, const uint32_t* __restrict__ owner, uint32_t base,
+                        int t)
+{
+    const uint32_t b = base + blockIdx.x;
+    const BatchSlot<S>& sl = slots[owner[b]];
+    const uint32_t lb = b - sl.block0;
+    const int bx = (int)(lb % (uint32_t)sl.nbx);
+    const int bz = (int)(lb / (uint32_t)sl.nbx);
+    const uint32_t seed = SR ? step_seed(sl.cfg_seed, sl.steps0 + t) : 0u;
+    compact_cells<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>(sl.args[(sl.par0 + t) & 1], bx, bz, seed);
 }
 
 double clock_now()
@@ -139,16 +170,21 @@ struct Instance {
     int state = BGKS_QUEUED;
     bool has_mem = false;
     size_t off = 0, len = 0;
+    int flag = -1;      // non-finite flag word while admitted
+    int skipped = 0;    // chunks passed over in a row (queued or active)
     double diag[5] = {0, 0, 0, 0, 0};
     int unstable = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
 
-int group_of(int storage, int rounding)
+// Launch group of a request: storage type and rounding (which select the
+// kernel's template), plus 8 for the compact layout.
+int group_of(int storage, int rounding, bool compact)
 {
     const bool sr = rounding && (storage == BGK_F16 || storage == BGK_BF16);
-    return storage * 2 + (sr ? 1 : 0);
+    return storage * 2 + (sr ? 1 : 0) + (compact ? 8 : 0);
 }
+constexpr int kGroups = 16;
 
 bgk_config request_config(const bgks_request& r, int device)
 {
@@ -169,7 +205,13 @@ bgk_config request_config(const bgks_request& r, int device)
 struct bgks_server {
     int device = 0;
     int chunk = 100;
-    hipStream_t stream = nullptr;
+    double chunk_ms = 0;          // chunk-time target (0: chunk_steps only)
+    double ns_per_cell_step = 0;  // measured by the last chunk
+    hipStream_t stream = nullptr, stream2 = nullptr;
+    hipEvent_t ready = nullptr;   // uploads of a chunk done (stream2 waits for it)
+    uint32_t* d_flags = nullptr;  // non-finite flags, one word per admitted request
+    uint32_t* h_flags = nullptr;
+    std::vector<int> free_flags;
     void* arena_mem = nullptr;
     Arena arena;
     double* scratch = nullptr;
@@ -186,7 +228,7 @@ struct bgks_server {
         void* h_slots = nullptr;
         uint32_t* h_owner = nullptr;
         size_t slot_bytes_cap = 0, owner_cap = 0;
-    } groups[8];
+    } groups[kGroups];
     long long launches = 0, cell_steps = 0, chunks = 0;
 };
 
@@ -225,7 +267,7 @@ struct SubBatch {
 
 template <typename S>
 int prepare_group(bgks_server& s, bgks_server::Group& g, const std::vector<Instance*>& members,
-                  std::vector<SubBatch>& subs)
+                  std::vector<SubBatch>& subs, hipStream_t stream)
 {
     std::vector<BatchSlot<S>> slots(members.size());
     uint32_t total = 0;
@@ -241,13 +283,15 @@ int prepare_group(bgks_server& s, bgks_server::Group& g, const std::vector<Insta
         BatchSlot<S>& sl = slots[q];
         sl.args[0] = fused_args<S>(h, 0);
         sl.args[1] = fused_args<S>(h, 1);
+        sl.args[0].bad = sl.args[1].bad = s.d_flags + members[q]->flag;
         sl.rm = row_map_capped(h.cfg.l, kServeWaves);
         sl.gy = ceil_div(h.cfg.m, sl.rm.rpb);
+        sl.nbx = ceil_div(sl.args[0].lend, 256);
         sl.par0 = h.cur;
         sl.cfg_seed = h.cfg.seed;
         sl.steps0 = h.steps;
         sl.block0 = total;
-        const long long nb = (long long)sl.rm.bpr * sl.gy * h.cfg.n;
+        const long long nb = h.compact ? (long long)sl.nbx * h.cfg.n : (long long)sl.rm.bpr * sl.gy * h.cfg.n;
         if ((long long)total + nb > 0xffffffffLL) return fail(BGK_EINVAL, "batch grid exceeds 2^32 blocks");
         if ((long long)subs.back().nblocks + nb > 0x7fffffffLL)
             return fail(BGK_EINVAL, "sub-batch grid exceeds 2^31 blocks");
@@ -262,22 +306,26 @@ int prepare_group(bgks_server& s, bgks_server::Group& g, const std::vector<Insta
         std::fill(g.h_owner + b0, g.h_owner + b1, (uint32_t)q);
     }
     ENG_HIP(hipMemcpyAsync(g.d_slots, g.h_slots, slots.size() * sizeof(BatchSlot<S>), hipMemcpyHostToDevice,
-                           s.stream));
-    ENG_HIP(hipMemcpyAsync(g.d_owner, g.h_owner, (size_t)total * sizeof(uint32_t), hipMemcpyHostToDevice, s.stream));
+                           stream));
+    ENG_HIP(hipMemcpyAsync(g.d_owner, g.h_owner, (size_t)total * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     return 0;
 }
 
 template <typename S, int SR>
-void launch_group(bgks_server& s, bgks_server::Group& g, const SubBatch& sb, int t)
+void launch_group(bgks_server::Group& g, const SubBatch& sb, int t, hipStream_t stream, bool compact)
 {
-    hipLaunchKernelGGL((collide_batched<S, SR>), dim3(sb.nblocks), dim3(64 * kServeWaves), 0, s.stream,
-                       static_cast<const BatchSlot<S>*>(g.d_slots), g.d_owner, sb.block0, t);
+    if (compact)
+        hipLaunchKernelGGL((collide_batched_compact<S, SR>), dim3(sb.nblocks), dim3(256), 0, stream,
+                           static_cast<const BatchSlot<S>*>(g.d_slots), g.d_owner, sb.block0, t);
+    else
+        hipLaunchKernelGGL((collide_batched<S, SR>), dim3(sb.nblocks), dim3(64 * kServeWaves), 0, stream,
+                           static_cast<const BatchSlot<S>*>(g.d_slots), g.d_owner, sb.block0, t);
 }
 
 template <typename F>
 int by_group(int gid, F&& f)
 {
-    switch (gid) {
+    switch (gid & 7) {
         case 0: return f(float{}, std::integral_constant<int, 0>{});
         case 2: return f(double{}, std::integral_constant<int, 0>{});
         case 4: return f(f16_t{}, std::integral_constant<int, 0>{});
@@ -300,17 +348,24 @@ int start(bgks_server& s, Instance& in, size_t off)
     void* region = static_cast<char*>(s.arena_mem) + off;
     place_lattices(in.h, region);
     ENG_HIP(hipMemsetAsync(region, 0, in.len, s.stream));  // rest state: shifted populations are zero
+    in.flag = s.free_flags.back();
+    s.free_flags.pop_back();
+    ENG_HIP(hipMemsetAsync(s.d_flags + in.flag, 0, sizeof(uint32_t), s.stream));
     in.state = BGKS_ACTIVE;
+    in.skipped = 0;
     in.t_admit = clock_now();
     return 0;
 }
 
 // Arrival order: the head of the queue waits for room.  Shortest first:
-// queued requests in order of their work, each admitted if it fits.
+// queued requests in order of their work, each admitted if it fits, except
+// that requests passed over kMaxSkips times come first, in arrival order,
+// and the first of them that does not fit stops admission (it waits for
+// room instead of being overtaken again).
 int admit(bgks_server& s)
 {
     if (s.policy == BGKS_ALL_ACTIVE) {
-        while (!s.queue.empty()) {
+        while (!s.queue.empty() && !s.free_flags.empty()) {
             Instance& in = *s.all.at(s.queue.front());
             size_t off = 0;
             if (!s.arena.alloc(in.len, off)) break;
@@ -321,12 +376,20 @@ int admit(bgks_server& s)
     }
     std::vector<long long> order(s.queue.begin(), s.queue.end());
     std::stable_sort(order.begin(), order.end(), [&](long long a, long long b) {
-        return remaining_work(*s.all.at(a)) < remaining_work(*s.all.at(b));
+        const Instance &x = *s.all.at(a), &y = *s.all.at(b);
+        const bool ax = x.skipped >= kMaxSkips, ay = y.skipped >= kMaxSkips;
+        if (ax != ay) return ax;
+        if (ax) return false;  // both aged: arrival order
+        return remaining_work(x) < remaining_work(y);
     });
     for (long long id : order) {
+        if (s.free_flags.empty()) break;
         Instance& in = *s.all.at(id);
         size_t off = 0;
-        if (!s.arena.alloc(in.len, off)) continue;
+        if (!s.arena.alloc(in.len, off)) {
+            if (in.skipped >= kMaxSkips) break;
+            continue;
+        }
         if (int rc = start(s, in, off)) return rc;
         s.queue.erase(std::find(s.queue.begin(), s.queue.end(), id));
     }
@@ -335,6 +398,10 @@ int admit(bgks_server& s)
 
 void free_lattice(bgks_server& s, Instance& in)
 {
+    if (in.flag >= 0) {
+        s.free_flags.push_back(in.flag);
+        in.flag = -1;
+    }
     if (!in.has_mem) return;
     s.arena.release(in.off, in.len);
     in.has_mem = false;
@@ -342,6 +409,38 @@ void free_lattice(bgks_server& s, Instance& in)
 }
 
 }  // namespace
+
+namespace bgk {
+namespace eng {
+
+// Shortest remaining work first with aging: every request passed over
+// max_skips chunks in a row, then the others in order of remaining work
+// (stable) until min_cells cells per step are in flight (at least one).
+std::vector<int> plan_shortest(const long long* remaining, const long long* cells, const int* skipped, int count,
+                               long long min_cells, int max_skips)
+{
+    std::vector<int> order(count), out;
+    for (int q = 0; q < count; ++q) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return remaining[a] < remaining[b]; });
+    long long flight = 0;
+    std::vector<char> taken(count, 0);
+    for (int q = 0; q < count; ++q)
+        if (skipped[q] >= max_skips) {
+            out.push_back(q);
+            taken[q] = 1;
+            flight += cells[q];
+        }
+    for (int q : order) {
+        if (taken[q]) continue;
+        if (!out.empty() && flight >= min_cells) break;
+        out.push_back(q);
+        flight += cells[q];
+    }
+    return out;
+}
+
+}  // namespace eng
+}  // namespace bgk
 
 extern "C" {
 
@@ -374,8 +473,14 @@ int bgks_create(int device, size_t budget_bytes, int chunk_steps, bgks_server** 
         bgks_destroy(s);
         return code;
     };
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ready, hipEventDisableTiming) != hipSuccess)
         return bail(fail(BGK_EHIP, "stream creation failed"));
+    if (hipMalloc(&s->d_flags, kMaxFlags * sizeof(uint32_t)) != hipSuccess ||
+        hipHostMalloc(&s->h_flags, kMaxFlags * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(BGK_ENOMEM, "flags"));
+    for (int q = kMaxFlags - 1; q >= 0; --q) s->free_flags.push_back(q);
     if (hipMalloc(&s->arena_mem, budget_bytes) != hipSuccess)
         return bail(fail(BGK_ENOMEM, "cannot reserve %.2f GB for the request arena", budget_bytes / 1e9));
     s->arena.reset(budget_bytes);
@@ -383,6 +488,25 @@ int bgks_create(int device, size_t budget_bytes, int chunk_steps, bgks_server** 
         hipHostMalloc(&s->hscratch, (kRedBlocks + 8) * 5 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(fail(BGK_ENOMEM, "scratch"));
     *out = s;
+    return 0;
+}
+
+int bgks_set_chunk_time(bgks_server* s, double target_ms)
+{
+    if (!s) return fail(BGK_EINVAL, "null server");
+    if (!(target_ms >= 0.0)) return fail(BGK_EINVAL, "chunk time %g ms", target_ms);
+    s->chunk_ms = target_ms;
+    return 0;
+}
+
+int bgks_schedule(const long long* remaining, const long long* cells, const int* skipped, int count,
+                  long long min_cells_per_step, int max_skips, int* selected, int* n_selected)
+{
+    if (count < 0 || (count > 0 && (!remaining || !cells || !skipped || !selected)) || !n_selected)
+        return fail(BGK_EINVAL, "bad arguments");
+    const std::vector<int> sel = plan_shortest(remaining, cells, skipped, count, min_cells_per_step, max_skips);
+    for (size_t q = 0; q < sel.size(); ++q) selected[q] = sel[q];
+    *n_selected = (int)sel.size();
     return 0;
 }
 
@@ -407,10 +531,15 @@ int bgks_destroy(bgks_server* s)
         if (g.h_slots) hipHostFree(g.h_slots);
         if (g.h_owner) hipHostFree(g.h_owner);
     }
+    if (s->stream2) hipStreamSynchronize(s->stream2);
     if (s->arena_mem) hipFree(s->arena_mem);
     if (s->scratch) hipFree(s->scratch);
     if (s->hscratch) hipHostFree(s->hscratch);
+    if (s->d_flags) hipFree(s->d_flags);
+    if (s->h_flags) hipHostFree(s->h_flags);
+    if (s->ready) hipEventDestroy(s->ready);
     if (s->stream) hipStreamDestroy(s->stream);
+    if (s->stream2) hipStreamDestroy(s->stream2);
     delete s;
     return 0;
 }
@@ -449,65 +578,108 @@ int bgks_advance(bgks_server* s, long long* done_ids, int max_done, int* n_done)
     for (auto& kv : s->all)
         if (kv.second->state == BGKS_ACTIVE) active.push_back(kv.second.get());
     if (s->policy == BGKS_SHORTEST_FIRST && !active.empty()) {
-        // the admitted requests with the least work left, enough of them to
-        // keep at least min_cells cells per step in flight
-        std::stable_sort(active.begin(), active.end(), [](const Instance* a, const Instance* b) {
-            return remaining_work(*a) < remaining_work(*b);
-        });
-        long long cells = 0;
-        size_t take = 0;
-        while (take < active.size() && (take == 0 || cells < s->min_cells)) {
-            const Instance* in = active[take++];
-            cells += (long long)in->h.cfg.l * in->h.cfg.m * in->h.cfg.n;
+        std::vector<long long> rem(active.size()), cells(active.size());
+        std::vector<int> skip(active.size());
+        for (size_t q = 0; q < active.size(); ++q) {
+            rem[q] = remaining_work(*active[q]);
+            cells[q] = (long long)active[q]->h.cfg.l * active[q]->h.cfg.m * active[q]->h.cfg.n;
+            skip[q] = active[q]->skipped;
         }
-        active.resize(take);
+        const std::vector<int> sel =
+            plan_shortest(rem.data(), cells.data(), skip.data(), (int)active.size(), s->min_cells, kMaxSkips);
+        std::vector<char> chosen(active.size(), 0);
+        for (int q : sel) chosen[q] = 1;
+        std::vector<Instance*> run;
+        for (size_t q = 0; q < active.size(); ++q) {
+            if (chosen[q]) {
+                active[q]->skipped = 0;
+                run.push_back(active[q]);
+            } else {
+                ++active[q]->skipped;
+            }
+        }
+        active.swap(run);
     }
+    for (long long id : s->queue) ++s->all.at(id)->skipped;
     long long K = s->chunk;
-    for (Instance* in : active) K = std::min(K, in->req.steps - in->h.steps);
+    long long cells_in_flight = 0;
+    for (Instance* in : active) {
+        K = std::min(K, in->req.steps - in->h.steps);
+        cells_in_flight += (long long)in->h.cfg.l * in->h.cfg.m * in->h.cfg.n;
+    }
+    // chunk-time target: as many steps as the last chunk's rate per cell
+    // step fits into chunk_ms (at least one)
+    if (s->chunk_ms > 0 && s->ns_per_cell_step > 0 && cells_in_flight > 0) {
+        const double per_step_ms = s->ns_per_cell_step * (double)cells_in_flight * 1e-6;
+        K = std::min(K, std::max(1LL, (long long)(s->chunk_ms / per_step_ms)));
+    }
     K = std::max(K, 0LL);
 
+    bool ran = false;
     if (K > 0 && !active.empty()) {
-        std::vector<Instance*> members[8];
-        for (Instance* in : active) members[group_of(in->h.cfg.storage, in->h.cfg.rounding)].push_back(in);
-        std::vector<SubBatch> subs[8];
-        for (int g = 0; g < 8; ++g) {
+        std::vector<Instance*> members[kGroups];
+        for (Instance* in : active)
+            members[group_of(in->h.cfg.storage, in->h.cfg.rounding, in->h.compact)].push_back(in);
+        std::vector<SubBatch> subs[kGroups];
+        // groups alternate between the two streams; every upload and the
+        // admissions' memsets are on the first stream, which the second
+        // waits for before its first launch
+        hipStream_t gstream[kGroups];
+        int ng = 0;
+        for (int g = 0; g < kGroups; ++g) {
             if (members[g].empty()) continue;
+            gstream[g] = (ng++ % 2) ? s->stream2 : s->stream;
             int rc = by_group(g, [&](auto tag, auto) {
-                return prepare_group<decltype(tag)>(*s, s->groups[g], members[g], subs[g]);
+                return prepare_group<decltype(tag)>(*s, s->groups[g], members[g], subs[g], s->stream);
             });
             if (rc) return rc;
         }
-        for (int g = 0; g < 8; ++g)
-            for (const SubBatch& sb : subs[g])
-                for (long long t = 0; t < K; ++t) {
+        ENG_HIP(hipEventRecord(s->ready, s->stream));
+        ENG_HIP(hipStreamWaitEvent(s->stream2, s->ready, 0));
+        const double t0 = clock_now();
+        for (long long t = 0; t < K; ++t)
+            for (int g = 0; g < kGroups; ++g)
+                for (const SubBatch& sb : subs[g]) {
                     by_group(g, [&](auto tag, auto sr) {
-                        launch_group<decltype(tag), decltype(sr)::value>(*s, s->groups[g], sb, (int)t);
+                        launch_group<decltype(tag), decltype(sr)::value>(s->groups[g], sb, (int)t, gstream[g],
+                                                                          g >= 8);
                         return 0;
                     });
                     ++s->launches;
                 }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(BGK_EHIP, "batched step launch: %s", hipGetErrorString(e));
+        ENG_HIP(hipStreamSynchronize(s->stream2));
+        ENG_HIP(hipStreamSynchronize(s->stream));
+        s->ns_per_cell_step = (clock_now() - t0) * 1e9 / ((double)K * (double)cells_in_flight);
         for (Instance* in : active) {
             in->h.steps += K;
             in->h.cur = (int)((in->h.cur + K) & 1);
             s->cell_steps += (long long)in->h.cfg.l * in->h.cfg.m * in->h.cfg.n * K;
         }
         ++s->chunks;
+        ran = true;
+        ENG_HIP(hipMemcpy(s->h_flags, s->d_flags, kMaxFlags * sizeof(uint32_t), hipMemcpyDeviceToHost));
     }
     ENG_HIP(hipStreamSynchronize(s->stream));
     const double now = clock_now();
 
     int nd = 0;
     for (Instance* in : active) {
-        if (in->h.steps < in->req.steps) continue;
+        // a non-finite value anywhere in the lattice retires the request now
+        const bool bad = ran && s->h_flags[in->flag] != 0;
+        if (!bad && in->h.steps < in->req.steps) continue;
         if (int rc = bgk_diagno(&in->h, in->diag)) return rc;
-        in->unstable = 0;
+        in->unstable = bad ? 1 : 0;
         for (double v : in->diag)
             if (!std::isfinite(v)) in->unstable = 1;
         in->state = BGKS_DONE;
         in->t_done = now;
         if (!in->req.keep) free_lattice(*s, *in);
+        else if (in->flag >= 0) {
+            s->free_flags.push_back(in->flag);
+            in->flag = -1;
+        }
         if (done_ids && nd < max_done) done_ids[nd] = in->id;
         ++nd;
     }

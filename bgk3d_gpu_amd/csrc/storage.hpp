@@ -39,6 +39,23 @@ struct f16r_t {
 
 enum StorageKind : int { ST_F32 = 0, ST_F64 = 1, ST_F16 = 2, ST_BF16 = 3, ST_MIX32 = 4, ST_F16R = 5 };
 
+// Every lattice lives in device (global) memory.  Loads and stores go
+// through a pointer in that address space, so they compile to global memory
+// instructions even where the pointer itself was read from memory (the
+// serving kernel's per-request tables), which the compiler would otherwise
+// treat as generic (flat) addresses.
+#define BGK_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const BGK_GLOBAL T* gmem(const T* p)
+{
+    return (const BGK_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ BGK_GLOBAL T* gmem(T* p)
+{
+    return (BGK_GLOBAL T*)p;
+}
+
 // load / load_nt: a stored value in the collision kind (plain or streaming
 // read); widen: the same for a value in a register.  lean: the 16-bit lean
 // collide (bgk_collide_lean) instead of the reference-order one.
@@ -50,8 +67,8 @@ struct Storage<float> {
     typedef float compute;
     static constexpr int kind = ST_F32;
     static constexpr bool lean = false;
-    __device__ __forceinline__ static float load(const float* p) { return *p; }
-    __device__ __forceinline__ static float load_nt(const float* p) { return __builtin_nontemporal_load(p); }
+    __device__ __forceinline__ static float load(const float* p) { return *gmem(p); }
+    __device__ __forceinline__ static float load_nt(const float* p) { return __builtin_nontemporal_load(gmem(p)); }
     __device__ __forceinline__ static float widen(float v) { return v; }
 };
 template <>
@@ -59,8 +76,8 @@ struct Storage<double> {
     typedef double compute;
     static constexpr int kind = ST_F64;
     static constexpr bool lean = false;
-    __device__ __forceinline__ static double load(const double* p) { return *p; }
-    __device__ __forceinline__ static double load_nt(const double* p) { return __builtin_nontemporal_load(p); }
+    __device__ __forceinline__ static double load(const double* p) { return *gmem(p); }
+    __device__ __forceinline__ static double load_nt(const double* p) { return __builtin_nontemporal_load(gmem(p)); }
     __device__ __forceinline__ static double widen(double v) { return v; }
 };
 template <>
@@ -68,8 +85,8 @@ struct Storage<f16_t> {
     typedef float compute;
     static constexpr int kind = ST_F16;
     static constexpr bool lean = true;
-    __device__ __forceinline__ static float load(const f16_t* p) { return (float)*p; }
-    __device__ __forceinline__ static float load_nt(const f16_t* p) { return (float)__builtin_nontemporal_load(p); }
+    __device__ __forceinline__ static float load(const f16_t* p) { return (float)*gmem(p); }
+    __device__ __forceinline__ static float load_nt(const f16_t* p) { return (float)__builtin_nontemporal_load(gmem(p)); }
     __device__ __forceinline__ static float widen(f16_t v) { return (float)v; }
 };
 template <>
@@ -79,11 +96,11 @@ struct Storage<bf16_t> {
     static constexpr bool lean = true;
     __device__ __forceinline__ static float load(const bf16_t* p)
     {
-        return __uint_as_float((uint32_t)p->u << 16);
+        return __uint_as_float((uint32_t)gmem(p)->u << 16);
     }
     __device__ __forceinline__ static float load_nt(const bf16_t* p)
     {
-        return __uint_as_float((uint32_t)__builtin_nontemporal_load(&p->u) << 16);
+        return __uint_as_float((uint32_t)__builtin_nontemporal_load(&gmem(p)->u) << 16);
     }
     __device__ __forceinline__ static float widen(bf16_t v) { return __uint_as_float((uint32_t)v.u << 16); }
 };
@@ -92,10 +109,10 @@ struct Storage<mix32_t> {
     typedef double compute;
     static constexpr int kind = ST_MIX32;
     static constexpr bool lean = false;
-    __device__ __forceinline__ static double load(const mix32_t* p) { return (double)p->v; }
+    __device__ __forceinline__ static double load(const mix32_t* p) { return (double)gmem(p)->v; }
     __device__ __forceinline__ static double load_nt(const mix32_t* p)
     {
-        return (double)__builtin_nontemporal_load(&p->v);
+        return (double)__builtin_nontemporal_load(&gmem(p)->v);
     }
     __device__ __forceinline__ static double widen(mix32_t v) { return (double)v.v; }
 };
@@ -104,10 +121,10 @@ struct Storage<f16r_t> {
     typedef float compute;
     static constexpr int kind = ST_F16R;
     static constexpr bool lean = false;
-    __device__ __forceinline__ static float load(const f16r_t* p) { return (float)p->v; }
+    __device__ __forceinline__ static float load(const f16r_t* p) { return (float)gmem(p)->v; }
     __device__ __forceinline__ static float load_nt(const f16r_t* p)
     {
-        return (float)__builtin_nontemporal_load(&p->v);
+        return (float)__builtin_nontemporal_load(&gmem(p)->v);
     }
     __device__ __forceinline__ static float widen(f16r_t v) { return (float)v.v; }
 };
@@ -199,23 +216,29 @@ struct Rounder<f16r_t, 1> {
 // Stores (plain and non-temporal); bf16_t and the mixed types are stored
 // through their member.
 template <typename S>
-__device__ __forceinline__ void store_value(S* p, S v) { *p = v; }
+__device__ __forceinline__ void store_value(S* p, S v) { *gmem(p) = v; }
 template <>
-__device__ __forceinline__ void store_value<bf16_t>(bf16_t* p, bf16_t v) { reinterpret_cast<uint16_t*>(p)[0] = v.u; }
+__device__ __forceinline__ void store_value<bf16_t>(bf16_t* p, bf16_t v) { gmem(p)->u = v.u; }
 template <typename S>
-__device__ __forceinline__ void store_nt(S* p, S v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_nt(S* p, S v) { __builtin_nontemporal_store(v, gmem(p)); }
 template <>
 __device__ __forceinline__ void store_nt<bf16_t>(bf16_t* p, bf16_t v)
 {
-    __builtin_nontemporal_store(v.u, reinterpret_cast<uint16_t*>(p));
+    __builtin_nontemporal_store(v.u, &gmem(p)->u);
 }
 template <>
-__device__ __forceinline__ void store_value<mix32_t>(mix32_t* p, mix32_t v) { p->v = v.v; }
+__device__ __forceinline__ void store_value<mix32_t>(mix32_t* p, mix32_t v) { gmem(p)->v = v.v; }
 template <>
-__device__ __forceinline__ void store_nt<mix32_t>(mix32_t* p, mix32_t v) { __builtin_nontemporal_store(v.v, &p->v); }
+__device__ __forceinline__ void store_nt<mix32_t>(mix32_t* p, mix32_t v)
+{
+    __builtin_nontemporal_store(v.v, &gmem(p)->v);
+}
 template <>
-__device__ __forceinline__ void store_value<f16r_t>(f16r_t* p, f16r_t v) { p->v = v.v; }
+__device__ __forceinline__ void store_value<f16r_t>(f16r_t* p, f16r_t v) { gmem(p)->v = v.v; }
 template <>
-__device__ __forceinline__ void store_nt<f16r_t>(f16r_t* p, f16r_t v) { __builtin_nontemporal_store(v.v, &p->v); }
+__device__ __forceinline__ void store_nt<f16r_t>(f16r_t* p, f16r_t v)
+{
+    __builtin_nontemporal_store(v.v, &gmem(p)->v);
+}
 
 }  // namespace bgk

@@ -26,7 +26,8 @@ class BGKConfig(ctypes.Structure):
                 ("obs_ic", ctypes.c_int), ("obs_jc", ctypes.c_int), ("obs_kc", ctypes.c_int),
                 ("obs_radius", ctypes.c_int), ("u_inflow", ctypes.c_double), ("force_axis", ctypes.c_int),
                 ("device", ctypes.c_int), ("seed", ctypes.c_uint), ("les", ctypes.c_int),
-                ("cte_s", ctypes.c_double), ("wall_axis", ctypes.c_int), ("mixed", ctypes.c_int)]
+                ("cte_s", ctypes.c_double), ("wall_axis", ctypes.c_int), ("mixed", ctypes.c_int),
+                ("layout", ctypes.c_int)]
 
 
 def lib() -> ctypes.CDLL:
@@ -66,9 +67,10 @@ def _check(rc, what):
 
 def make_config(case: Case, storage: str = "f32", rounding: int = 0, obstacle: int | None = None,
                 centre=None, radius=None, u_inflow=None, device=0, seed=0, les=None, cte_s=None,
-                mixed: int = 0) -> BGKConfig:
+                mixed: int = 0, layout: int = 0) -> BGKConfig:
     """mixed: 1 the reference's MIXED1 form (storage 'f16', reference-order
-    FP32 collision), 2 its MIXED2 form (storage 'f32', FP64 collision)."""
+    FP32 collision), 2 its MIXED2 form (storage 'f32', FP64 collision).
+    layout: 0 choose, 1 rows padded to 128-byte lines, 2 compact rows."""
     c = BGKConfig()
     lib().bgk_config_default(ctypes.byref(c), case.l, case.m, case.n)
     c.boundary = BOUNDARY[case.boundary]
@@ -90,6 +92,7 @@ def make_config(case: Case, storage: str = "f32", rounding: int = 0, obstacle: i
     c.cte_s = case.cte_s if cte_s is None else cte_s
     c.wall_axis = case.wall_axis
     c.mixed = mixed
+    c.layout = layout
     return c
 
 

@@ -2,9 +2,11 @@
 
 `Server` binds the continuous-batching server of include/bgk3d_engine.h
 (bgks_*): requests are admitted within an HBM budget (arrival order, or
-shortest remaining work first) and the scheduled requests advance with one
-kernel launch per storage group, step and sub-batch of at most 3 GB.  `run_batched` drives it over a request stream; `run_sequential` is
-the plain loop the reference's sweep script stands for
+shortest remaining work first with aging) and the scheduled requests
+advance with one kernel launch per storage group, step and sub-batch of at
+most 3 GB; a request whose solution turns non-finite is retired at the end
+of that chunk.  `run_batched` drives it over a request stream;
+`run_sequential` is the plain loop the reference's sweep script stands for
 (UTIL/run_sweep.sh:55-63, 128-169): one solve after the other, each with its
 own handle (allocate, rest start, `steps` time steps, diagnostics, free).
 
@@ -53,6 +55,9 @@ def lib() -> ctypes.CDLL:
         L.bgks_get_population.argtypes = [vp, ll, ci, vp]
         L.bgks_release.argtypes = [vp, ll]
         L.bgks_stats.argtypes = [vp, ctypes.POINTER(ll)]
+        L.bgks_set_chunk_time.argtypes = [vp, ctypes.c_double]
+        L.bgks_schedule.argtypes = [ctypes.POINTER(ll), ctypes.POINTER(ll), ctypes.POINTER(ci), ci, ll, ci,
+                                    ctypes.POINTER(ci), ctypes.POINTER(ci)]
         L.bgks_clock.argtypes = []
         L.bgks_clock.restype = ctypes.c_double
         L._bgks_typed = True
@@ -122,19 +127,33 @@ def make_workload(count: int, steps: int = 5000, rate: float | None = None, size
             for q, (n, re, h, a) in enumerate(zip(ns, res, half, arrivals))]
 
 
+def schedule(remaining, cells, skipped, min_cells: int, max_skips: int = 8) -> list[int]:
+    """The server's shortest-first selection (bgks_schedule): indices of the
+    requests that advance in the next chunk."""
+    n = len(remaining)
+    ll, ci = ctypes.c_longlong, ctypes.c_int
+    sel, ns = (ci * max(n, 1))(), ci()
+    E._check(lib().bgks_schedule((ll * max(n, 1))(*remaining), (ll * max(n, 1))(*cells), (ci * max(n, 1))(*skipped),
+                                 n, min_cells, max_skips, sel, ctypes.byref(ns)), "bgks_schedule")
+    return [sel[q] for q in range(ns.value)]
+
+
 class Server:
     """policy 'all': every admitted request advances each chunk, arrival-order
     admission; 'shortest': least remaining work first, enough requests to keep
-    `min_cells` cells per step in flight."""
+    `min_cells` cells per step in flight, with aging.  chunk_ms > 0 cuts
+    chunks to about that many milliseconds."""
 
     def __init__(self, budget_bytes: int, chunk_steps: int = 100, device: int = 0, policy: str = "all",
-                 min_cells: int = 0):
+                 min_cells: int = 0, chunk_ms: float = 0.0):
         self.L = lib()
         h = ctypes.c_void_p()
         E._check(self.L.bgks_create(device, budget_bytes, chunk_steps, ctypes.byref(h)), "bgks_create")
         self.h = h.value
         if policy != "all" or min_cells:
             E._check(self.L.bgks_set_policy(self.h, POLICY[policy], min_cells), "bgks_set_policy")
+        if chunk_ms:
+            E._check(self.L.bgks_set_chunk_time(self.h, chunk_ms), "bgks_set_chunk_time")
 
     def submit(self, r: Request) -> int:
         rid = ctypes.c_longlong()
@@ -194,14 +213,20 @@ class Server:
 
 
 def _summary(requests, records, t0, t_end, label):
+    """throughput_mlups: the requested cell updates of the stream over the
+    makespan (a request retired as unstable counts as served);
+    executed_mlups: the cell updates actually run over the makespan."""
     lat = np.array([rec["done"] - (t0 + r.arrival) for r, rec in zip(requests, records)])
     work = sum(r.cells * r.steps for r in requests)
+    ran = sum(r.cells * rec.get("steps_run", r.steps) for r, rec in zip(requests, records))
     return {
         "scheduler": label,
         "requests": len(requests),
         "cell_steps": work,
+        "cell_steps_run": ran,
         "makespan_s": t_end - t0,
         "throughput_mlups": work / (t_end - t0) / 1e6,
+        "executed_mlups": ran / (t_end - t0) / 1e6,
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_mean_s": float(lat.mean()),
@@ -218,13 +243,13 @@ def _wait_until(t):
 
 
 def run_batched(requests, budget_bytes: int, chunk_steps: int = 100, online: bool = False,
-                progress=None, policy: str = "all", min_cells: int = 0) -> dict:
+                progress=None, policy: str = "all", min_cells: int = 0, chunk_ms: float = 0.0) -> dict:
     """Serve `requests` with the batching server.  Offline: all submitted at
     the start.  Online: each submitted at its arrival time (checked between
     chunks, so a request waits at most one chunk to be queued)."""
     order = sorted(range(len(requests)), key=lambda q: requests[q].arrival)
     records = [None] * len(requests)
-    with Server(budget_bytes, chunk_steps, policy=policy, min_cells=min_cells) as srv:
+    with Server(budget_bytes, chunk_steps, policy=policy, min_cells=min_cells, chunk_ms=chunk_ms) as srv:
         ids = {}
         nxt = 0
         t0 = clock()
@@ -247,6 +272,7 @@ def run_batched(requests, budget_bytes: int, chunk_steps: int = 100, online: boo
             for rid in done:
                 q = ids[rid]
                 records[q] = srv.result(rid)
+                records[q]["steps_run"] = srv.status(rid)[1]
                 finished += 1
                 if not requests[q].keep:
                     srv.release(rid)
@@ -256,7 +282,7 @@ def run_batched(requests, budget_bytes: int, chunk_steps: int = 100, online: boo
         st = srv.stats()
     out = _summary(requests, records, t0, t_end, f"batched-{policy}")
     out.update(budget_bytes=budget_bytes, chunk_steps=chunk_steps, launches=st["launches"], chunks=st["chunks"],
-               peak_admitted=peak_admitted, policy=policy, min_cells=min_cells)
+               peak_admitted=peak_admitted, policy=policy, min_cells=min_cells, chunk_ms=chunk_ms)
     out["records"] = records
     return out
 
@@ -308,4 +334,4 @@ def run_sequential(requests, online: bool = False, progress=None, path: str = "e
 
 
 __all__ = ["Request", "Server", "make_workload", "run_batched", "run_sequential", "request_bytes", "clock",
-           "BGKSRequest"]
+           "BGKSRequest", "schedule"]

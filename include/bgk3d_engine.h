@@ -64,6 +64,10 @@ typedef struct bgk_config {
                              0 none; 1 MIXED1: storage BGK_F16, the collision in FP32 in the
                              reference's expression order (the plain F16 form uses a leaner
                              FP32 arithmetic); 2 MIXED2: storage BGK_F32, collision in FP64 */
+    int layout;           /* 0 choose, 1 rows padded to 128-byte lines, 2 compact rows of l + 2
+                             cells (not with BGK_BC_INFLOW).  The default pads the rows unless
+                             that leaves more than 12 % of a row unused in its last line (l just
+                             above a multiple of 32 values for FP32, 16 for FP64, 64 for 16-bit) */
 } bgk_config;
 
 typedef struct bgk_handle bgk_handle;
@@ -152,11 +156,15 @@ const char* bgk_last_error(void);
  * Requests are admitted while the device arena reserved by bgks_create (the
  * HBM budget) can hold their two lattices; the requests scheduled for a
  * chunk advance by one kernel launch per (storage, rounding) group, step
- * and sub-batch of at most 3 GB of lattices; requests join and leave
- * between chunks of at most chunk_steps steps (see enum bgks_policy).  A request's populations are bit-identical
- * to those of a handle (bgk_create with the same n, svisc, u0, storage,
- * rounding and seed) after the same number of steps, whatever else is
- * running.  Not thread-safe: one caller drives a server.
+ * and sub-batch of at most 3 GB of lattices (the groups on two streams, so
+ * their launches overlap); requests join and leave between chunks of at
+ * most chunk_steps steps, cut shorter by a chunk-time target if one is set
+ * (see enum bgks_policy).  A request whose populations turn non-finite (the
+ * step kernel checks every cell) is retired as unstable at the end of that
+ * chunk.  A request's populations are bit-identical to those of a handle
+ * (bgk_create with the same n, svisc, u0, storage, rounding and seed) after
+ * the same number of steps, whatever else is running.  Not thread-safe: one
+ * caller drives a server; at most 4096 requests are admitted at once.
  * ------------------------------------------------------------------------- */
 typedef struct bgks_server bgks_server;
 
@@ -177,7 +185,8 @@ enum bgks_state { BGKS_QUEUED = 0, BGKS_ACTIVE = 1, BGKS_DONE = 2 };
 enum bgks_policy {
     BGKS_ALL_ACTIVE = 0,     /* admit in arrival order; every admitted request advances each chunk */
     BGKS_SHORTEST_FIRST = 1  /* admit and advance the requests with the least work (cells x steps)
-                                left first, enough of them to keep min_cells_per_step in flight */
+                                left first, enough of them to keep min_cells_per_step in flight;
+                                a request passed over 8 chunks in a row goes first (aging) */
 };
 
 /* Reserve `budget_bytes` of device memory for request lattices
@@ -187,6 +196,21 @@ int bgks_destroy(bgks_server* s);
 
 /* Choose the scheduling policy (takes effect at the next bgks_advance). */
 int bgks_set_policy(bgks_server* s, int policy, long long min_cells_per_step);
+
+/* Cut chunks so that each takes about target_ms milliseconds (estimated from
+ * the previous chunk's time per cell update); 0 (the default) runs
+ * chunk_steps steps.  Requests join between chunks, so this bounds how long
+ * an arrival waits to start. */
+int bgks_set_chunk_time(bgks_server* s, double target_ms);
+
+/* The shortest-first selection the server makes each chunk, for hosts that
+ * schedule requests themselves: of `count` requests with remaining work
+ * (cells x steps), cells and chunks passed over in a row, every one passed
+ * over max_skips or more times, then the others by least remaining work
+ * (ties in index order) until min_cells_per_step cells are selected (at least
+ * one request).  selected[] receives the indices, *n_selected their number. */
+int bgks_schedule(const long long* remaining, const long long* cells, const int* skipped, int count,
+                  long long min_cells_per_step, int max_skips, int* selected, int* n_selected);
 
 /* Device bytes a request holds while admitted (both lattices, padded). */
 int bgks_request_bytes(const bgks_request* r, size_t* bytes);
@@ -205,7 +229,8 @@ int bgks_status(bgks_server* s, long long id, int* state, long long* steps_done)
 
 /* A finished request: diagnostics means as bgk_diagno, the times (seconds
  * on the bgks_clock scale) of submission, admission and completion, and
- * whether the solution became non-finite (unstable). */
+ * whether the solution became non-finite (unstable; such a request is
+ * retired when that is detected, bgks_status giving the steps it ran). */
 int bgks_result(bgks_server* s, long long id, double diag[5], double times[3], int* unstable);
 
 /* Interior values of population d of a request that holds its lattice. */

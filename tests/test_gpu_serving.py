@@ -205,3 +205,49 @@ def test_served_requests_reproduce_reference_runs(gpu):
         assert not res[q]["unstable"]
     for q in range(len(keys), len(reqs) + len(fillers)):
         assert not res[q]["unstable"]
+
+
+def test_unstable_request_is_retired_early(gpu):
+    """A cavity at tau = 0.5 + 3e-5 with lid speed 0.1 blows up within a few
+    hundred steps: the step kernel flags its non-finite cells and the server
+    retires it as unstable at the end of that chunk, long before its 20,000
+    steps; the stable requests it shared chunks with end exactly as their
+    alone runs, FP32 and FP16 groups running on the two streams."""
+    bad = S.Request(48, 0.0, "f32", 20000, seed=5, keep=False, nu=1e-5)
+    good = [S.Request(40, 400.0, "f16", 600, seed=6, rounding=1, keep=True),
+            S.Request(56, 300.0, "f32", 500, seed=7, keep=True)]
+    banner("serving: retire a non-finite request early")
+    with S.Server(4 << 30, 50) as srv:
+        bid = srv.submit(bad)
+        gids = [srv.submit(r) for r in good]
+        done = []
+        while len(done) < 3:
+            done += srv.advance()
+        st, steps = srv.status(bid)
+        res = srv.result(bid)
+        pops = [srv.populations(i, r.n) for i, r in zip(gids, good)]
+        gres = [srv.result(i) for i in gids]
+    assert st == "done" and res["unstable"]
+    assert 0 < steps < 5000, steps
+    assert done.index(bid) < len(done) - 1 or steps < 600
+    for q, r in enumerate(good):
+        assert not gres[q]["unstable"]
+        assert np.array_equal(pops[q], alone(r)[0])
+
+
+def test_chunk_time_target_keeps_results(gpu):
+    """Chunks cut to a time target (50 us: after the first chunk one or a
+    few steps each) leave every request's populations bit-identical to its
+    alone run."""
+    reqs = mix_requests()[:6]
+    banner("serving with a 50 us chunk-time target")
+    with S.Server(8 << 30, 1000, chunk_ms=0.05) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        done = set()
+        while len(done) < len(ids):
+            done.update(srv.advance())
+        chunks = srv.stats()["chunks"]
+        pops = [srv.populations(i, r.n) for i, r in zip(ids, reqs)]
+    assert chunks > 20
+    for q, r in enumerate(reqs):
+        assert np.array_equal(pops[q], alone(r)[0], equal_nan=True), q

@@ -33,6 +33,9 @@ CASES = {
     # 2.5 waves of cells
     "periodic_rows": Case("periodic", 62, 10, 6, svisc=0.01, u0=0.05, init=1),
     "ldc_wide_pad": Case("ldc", 40, 6, 5, svisc=0.02, u0=0.1, ipad=100),
+    # rows longer than a wave, with row padding
+    "channel_wide": Case("channel", 70, 12, 10, svisc=0.1, u0=0.02048),
+    "periodic_wide_pad": Case("periodic", 64, 9, 7, svisc=0.01, u0=0.05, init=1, ipad=5, jpad=1),
 }
 
 
