@@ -7,20 +7,22 @@
 // (SRC/bgk3d.F90:169-196).  Here every admitted request is an engine handle
 // whose two lattices live in one device arena sized by the caller's HBM
 // budget; requests are admitted in arrival order while the arena can hold
-// them, and all active requests are advanced by one kernel launch per
-// (storage type, rounding) group and step.  The launch grid is the
-// concatenation of every request's row blocks; a block finds its request
-// through a per-block owner table and then runs the same per-cell code as
-// the single-request kernel, so a request's result does not depend on what
-// it shares the GPU with (stochastic-rounding streams are keyed by cell
-// index, request seed and the request's own step count).
+// them.  The requests of a chunk are split by (storage type, rounding, row
+// layout) group into sub-batches of at most 512 MB of lattices, and each
+// sub-batch runs all the chunk's steps, one launch per step, before the
+// next.  A request alone in its sub-batch runs the engine's own kernel; the
+// launch grid of a larger sub-batch is the concatenation of its requests'
+// row blocks, and a block finds its request through a per-block owner table
+// and then runs the same per-cell code as the single-request kernel.  So a
+// request's result does not depend on what it shares the GPU with
+// (stochastic-rounding streams are keyed by cell index, request seed and
+// the request's own step count).
 //
 // Steps run in chunks of at most `chunk_steps`, shortened so that no request
 // overshoots its step count and, with a chunk-time target, so that a chunk
 // takes about that long (requests join and leave only between chunks, so
-// the target bounds how long an arrival waits to start).  The storage
-// groups of a chunk run on two streams, step by step, so their launches
-// overlap.  Every cell's new rest population is checked for a non-finite
+// the target bounds how long an arrival waits to start).  Every cell's new
+// rest population is checked for a non-finite
 // value inside the step kernel; a request whose solution became non-finite
 // is retired as unstable at the end of the chunk instead of running to its
 // last step.
@@ -54,11 +56,15 @@ constexpr int kServeWaves = 6;
 constexpr size_t kArenaAlign = 64 << 10;
 // A chunk runs its steps sub-batch by sub-batch: consecutive requests of a
 // storage group whose lattices add up to at most this many bytes (one larger
-// request forms its own).  The fused kernel streams about 6 % faster over a
-// 1.5-3 GB working set than over 10-20 GB, whose address translations do not
-// stay cached from one launch to the next; every request still advances all
-// the chunk's steps in order, so results do not depend on the grouping.
-constexpr size_t kSubBatchBytes = size_t(3) << 30;
+// request forms its own), each sub-batch all K steps before the next.  The
+// fused kernel streams faster over a small working set, whose address
+// translations stay cached from one launch to the next (measured on the
+// 16-request cavity mix, FP32 part: 30.9k MLUPS with 512 MB sub-batches,
+// 28.1k with 3 GB).  Every request advances the chunk's steps in order, so
+// results do not depend on the grouping.  A request alone in its sub-batch
+// runs the engine's own kernel (no block-to-request lookup); batching in one
+// launch pays only for requests too small to fill the GPU by themselves.
+constexpr size_t kSubBatchBytes = size_t(1) << 29;
 // Requests admitted at once: each holds one non-finite flag word.
 constexpr int kMaxFlags = 4096;
 // Shortest-first aging: chunks a request may be passed over in a row.
@@ -207,8 +213,7 @@ struct bgks_server {
     int chunk = 100;
     double chunk_ms = 0;          // chunk-time target (0: chunk_steps only)
     double ns_per_cell_step = 0;  // measured by the last chunk
-    hipStream_t stream = nullptr, stream2 = nullptr;
-    hipEvent_t ready = nullptr;   // uploads of a chunk done (stream2 waits for it)
+    hipStream_t stream = nullptr;
     uint32_t* d_flags = nullptr;  // non-finite flags, one word per admitted request
     uint32_t* h_flags = nullptr;
     std::vector<int> free_flags;
@@ -263,6 +268,7 @@ int grow_group(bgks_server& s, bgks_server::Group& g, size_t slot_bytes, size_t 
 // step count at the chunk start) and the owner of every block.
 struct SubBatch {
     uint32_t block0, nblocks;
+    int single;  // the member that is alone in this sub-batch, else -1
 };
 
 template <typename S>
@@ -276,8 +282,10 @@ int prepare_group(bgks_server& s, bgks_server::Group& g, const std::vector<Insta
     for (size_t q = 0; q < members.size(); ++q) {
         bgk_handle& h = members[q]->h;
         if (subs.empty() || (sub_bytes > 0 && sub_bytes + members[q]->len > kSubBatchBytes)) {
-            subs.push_back({total, 0});
+            subs.push_back({total, 0, (int)q});
             sub_bytes = 0;
+        } else {
+            subs.back().single = -1;
         }
         sub_bytes += members[q]->len;
         BatchSlot<S>& sl = slots[q];
@@ -314,6 +322,21 @@ int prepare_group(bgks_server& s, bgks_server::Group& g, const std::vector<Insta
 template <typename S, int SR>
 void launch_group(bgks_server::Group& g, const SubBatch& sb, int t, hipStream_t stream, bool compact)
 {
+    if (sb.single >= 0) {
+        // the engine's launch for a driven cavity (engine.hip launch_fused)
+        const BatchSlot<S>& sl = static_cast<const BatchSlot<S>*>(g.h_slots)[sb.single];
+        FusedArgs<S> p = sl.args[(sl.par0 + t) & 1];
+        p.seed = step_seed(sl.cfg_seed, sl.steps0 + t);
+        if (compact) {
+            hipLaunchKernelGGL((collide_compact<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>),
+                               dim3((unsigned)ceil_div(p.lend, 256), (unsigned)p.n), dim3(256), 0, stream, p);
+        } else {
+            const RowMap rm = row_map(p.l);
+            hipLaunchKernelGGL((collide_fused<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>), fused_grid(rm, p.m, p.n),
+                               fused_block(rm), 0, stream, p, rm);
+        }
+        return;
+    }
     if (compact)
         hipLaunchKernelGGL((collide_batched_compact<S, SR>), dim3(sb.nblocks), dim3(256), 0, stream,
                            static_cast<const BatchSlot<S>*>(g.d_slots), g.d_owner, sb.block0, t);
@@ -473,9 +496,7 @@ int bgks_create(int device, size_t budget_bytes, int chunk_steps, bgks_server** 
         bgks_destroy(s);
         return code;
     };
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&s->ready, hipEventDisableTiming) != hipSuccess)
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(BGK_EHIP, "stream creation failed"));
     if (hipMalloc(&s->d_flags, kMaxFlags * sizeof(uint32_t)) != hipSuccess ||
         hipHostMalloc(&s->h_flags, kMaxFlags * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
@@ -531,15 +552,12 @@ int bgks_destroy(bgks_server* s)
         if (g.h_slots) hipHostFree(g.h_slots);
         if (g.h_owner) hipHostFree(g.h_owner);
     }
-    if (s->stream2) hipStreamSynchronize(s->stream2);
     if (s->arena_mem) hipFree(s->arena_mem);
     if (s->scratch) hipFree(s->scratch);
     if (s->hscratch) hipHostFree(s->hscratch);
     if (s->d_flags) hipFree(s->d_flags);
     if (s->h_flags) hipHostFree(s->h_flags);
-    if (s->ready) hipEventDestroy(s->ready);
     if (s->stream) hipStreamDestroy(s->stream);
-    if (s->stream2) hipStreamDestroy(s->stream2);
     delete s;
     return 0;
 }
@@ -621,27 +639,23 @@ int bgks_advance(bgks_server* s, long long* done_ids, int max_done, int* n_done)
         for (Instance* in : active)
             members[group_of(in->h.cfg.storage, in->h.cfg.rounding, in->h.compact)].push_back(in);
         std::vector<SubBatch> subs[kGroups];
-        // groups alternate between the two streams; every upload and the
-        // admissions' memsets are on the first stream, which the second
-        // waits for before its first launch
-        hipStream_t gstream[kGroups];
-        int ng = 0;
         for (int g = 0; g < kGroups; ++g) {
             if (members[g].empty()) continue;
-            gstream[g] = (ng++ % 2) ? s->stream2 : s->stream;
             int rc = by_group(g, [&](auto tag, auto) {
                 return prepare_group<decltype(tag)>(*s, s->groups[g], members[g], subs[g], s->stream);
             });
             if (rc) return rc;
         }
-        ENG_HIP(hipEventRecord(s->ready, s->stream));
-        ENG_HIP(hipStreamWaitEvent(s->stream2, s->ready, 0));
         const double t0 = clock_now();
-        for (long long t = 0; t < K; ++t)
-            for (int g = 0; g < kGroups; ++g)
-                for (const SubBatch& sb : subs[g]) {
+        // each sub-batch runs all K steps before the next one, so that
+        // consecutive launches stay inside its lattices.  One stream: two
+        // sub-batches streaming side by side on a second stream ran the mix
+        // about 6 % slower than one after the other.
+        for (int g = 0; g < kGroups; ++g)
+            for (const SubBatch& sb : subs[g])
+                for (long long t = 0; t < K; ++t) {
                     by_group(g, [&](auto tag, auto sr) {
-                        launch_group<decltype(tag), decltype(sr)::value>(s->groups[g], sb, (int)t, gstream[g],
+                        launch_group<decltype(tag), decltype(sr)::value>(s->groups[g], sb, (int)t, s->stream,
                                                                           g >= 8);
                         return 0;
                     });
@@ -649,7 +663,6 @@ int bgks_advance(bgks_server* s, long long* done_ids, int max_done, int* n_done)
                 }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(BGK_EHIP, "batched step launch: %s", hipGetErrorString(e));
-        ENG_HIP(hipStreamSynchronize(s->stream2));
         ENG_HIP(hipStreamSynchronize(s->stream));
         s->ns_per_cell_step = (clock_now() - t0) * 1e9 / ((double)K * (double)cells_in_flight);
         for (Instance* in : active) {

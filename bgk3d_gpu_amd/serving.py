@@ -230,6 +230,7 @@ def _summary(requests, records, t0, t_end, label):
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_mean_s": float(lat.mean()),
+        "latencies_s": [float(x) for x in lat],
         "unstable": [q for q, rec in enumerate(records) if rec["unstable"]],
     }
 

@@ -155,9 +155,9 @@ const char* bgk_last_error(void);
  *
  * Requests are admitted while the device arena reserved by bgks_create (the
  * HBM budget) can hold their two lattices; the requests scheduled for a
- * chunk advance by one kernel launch per (storage, rounding) group, step
- * and sub-batch of at most 3 GB of lattices (the groups on two streams, so
- * their launches overlap); requests join and leave between chunks of at
+ * chunk advance sub-batch by sub-batch (at most 512 MB of lattices of one
+ * storage, rounding and row layout), one kernel launch per sub-batch and
+ * step, on one stream; requests join and leave between chunks of at
  * most chunk_steps steps, cut shorter by a chunk-time target if one is set
  * (see enum bgks_policy).  A request whose populations turn non-finite (the
  * step kernel checks every cell) is retired as unstable at the end of that
