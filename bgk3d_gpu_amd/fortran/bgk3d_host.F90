@@ -77,6 +77,17 @@
          if (rc /= 0) stop 'hipMalloc failed'
       enddo
       pb(19) = pa(19)
+! bgk_init_gpu writes the cells 0:l+1, 0:m+1, 0:n+1; the row and plane
+! padding (ipad, jpad > 0) is zeroed first so that out.bin is defined there
+      h = zero
+      do d = 1, 19
+         rc = bgk_memcpy_h2d(pa(d), ph(d), bytes)
+         if (rc /= 0) stop 'h2d copy failed'
+      enddo
+      do d = 1, 18
+         rc = bgk_memcpy_h2d(pb(d), ph(d), bytes)
+         if (rc /= 0) stop 'h2d copy failed'
+      enddo
       rc = bgk_init_gpu(pa, lx, ly, lz, nx, ny, mode, u00)
       if (rc /= 0) stop 'bgk_init_gpu failed'
       rc = bgk_init_gpu(pb, lx, ly, lz, nx, ny, mode, u00)

@@ -19,8 +19,13 @@ REPO = Path(__file__).resolve().parents[1]
 FIX = json.loads((REPO / "tests" / "golden" / "fixtures.json").read_text())
 
 
-@pytest.mark.parametrize("key", ["ldc_24_sp", "ldc_24_dp", "tgv_32_sp", "channel_64x32x32_dp"])
-def test_fortran_host_reproduces_reference(gpu, tmp_path, key):
+@pytest.mark.parametrize("key,ipad,jpad", [("ldc_24_sp", 0, 0), ("ldc_24_dp", 0, 0), ("tgv_32_sp", 0, 0),
+                                           ("channel_64x32x32_dp", 0, 0), ("ldc_24_sp", 5, 3),
+                                           ("tgv_32_sp", 1, 2)])
+def test_fortran_host_reproduces_reference(gpu, tmp_path, key, ipad, jpad):
+    """ipad, jpad > 0: the reference's row and plane padding
+    (nx = l + 2 + ipad, ny = m + 2 + jpad, SRC/alloca.F90:153); the result is the
+    same and the padding holds zeros."""
     spec = FIX["runs"][key]
     prec = spec["prec"]
     exe = REPO / "bgk3d_gpu_amd" / "fortran" / "bin" / f"bgk3d_host_{prec}.x"
@@ -29,8 +34,8 @@ def test_fortran_host_reproduces_reference(gpu, tmp_path, key):
         "&parameters\n"
         f"lx = {spec['l']}\nly = {spec['m']}\nlz = {spec['n']}\nsvisc={spec['svisc']}\nu0={spec['u0']}\n"
         f"itfin={spec['steps']}\nivtim=1000000\nisignal=1000000\nitsave=1000000\nicheck={spec['icheck']}\n"
-        f"irestart=0\ninit_v={spec['init']}\nipad=0 /\n")
-    print(f"[gpu] fortran host {key}", flush=True)
+        f"irestart=0\ninit_v={spec['init']}\nipad={ipad}\njpad={jpad} /\n")
+    print(f"[gpu] fortran host {key} ipad {ipad} jpad {jpad}", flush=True)
     sys.stdout.flush()
     out = subprocess.run([str(exe), str(tmp_path / "bgk.input"), spec["case"], str(tmp_path)], capture_output=True,
                          text=True, timeout=240)
@@ -38,8 +43,9 @@ def test_fortran_host_reproduces_reference(gpu, tmp_path, key):
     assert "# Mlups" in out.stdout
     l, m, n = spec["l"], spec["m"], spec["n"]
     dt = np.float32 if prec == "sp" else np.float64
-    A = np.fromfile(tmp_path / "out.bin", dtype=dt).reshape(19, n + 2, m + 2, l + 2)
-    I = np.ascontiguousarray(A[:, 1:-1, 1:-1, 1:-1])
+    A = np.fromfile(tmp_path / "out.bin", dtype=dt).reshape(19, n + 2, m + 2 + jpad, l + 2 + ipad)
+    assert not A[:, :, :, l + 2:].any() and not A[:, :, m + 2:, :].any()
+    I = np.ascontiguousarray(A[:, 1:-1, 1:m + 1, 1:l + 1])
     assert hashlib.sha256(I.tobytes()).hexdigest() == spec["sha256_interior"]
     # diagno.dat: same steps; values equal to print precision.  The means are
     # FP64 sums over cells (the device sums in a different order), so a mean

@@ -10,6 +10,12 @@
   below half an FP16 ulp of the populations and the flow stagnates; with
   stochastic rounding the velocity stays within 1e-3 (L2, relative) of the
   FP64 run -- the precision target of the FP16 storage form.
+* The Poiseuille channel at its full size (512 x 128 x 128, walls at
+  y = 1/2 and m + 1/2, g_x = 1e-6, tau = 0.8, 50,000 steps from rest):
+  FP16 storage with stochastic rounding within 1e-3 (L2, relative) of the
+  FP64 run, and the plane-averaged u_x of both within the stated error of
+  the analytic start-up solution (FP64 5.2e-5 measured: the lattice's own
+  discretisation error; FP16-SR 1.3e-4).
 """
 import sys
 
@@ -18,6 +24,7 @@ import pytest
 
 from bgk3d_gpu_amd.cases import Case
 from bgk3d_gpu_amd.engine import Engine
+from bgk3d_gpu_amd.studies.precision import analytic_ux, run as run_channel
 
 pytestmark = pytest.mark.gpu
 
@@ -123,3 +130,30 @@ def test_fp16_stochastic_rounding_meets_precision_target(gpu):
     assert err[("f16", 1)] < 1e-3          # the FP16-storage precision target
     assert err[("f16", 0)] > 0.1           # nearest rounding stagnates (documented)
     assert err[("bf16", 1)] < 5e-3
+
+
+def test_poiseuille_full_size_fp16_stochastic(gpu):
+    """Analytic solution: steady parabola of curvature -g/nu plus the decaying
+    Fourier series at t = 50,000 plus the g/2 that the stored post-collision
+    momentum carries (studies/precision.py analytic_ux)."""
+    l, m, n, steps, nu, g = 512, 128, 128, 50000, 0.1, 1e-6
+    case = Case("channel", l, m, n, svisc=nu, u0=(g / 10) * 5 * m * m / (4 * nu), wall_axis=2)
+    exact = analytic_ux(m, g, nu, steps)
+    out = {}
+    for storage, rnd in (("f64", 0), ("f16", 1)):
+        banner(f"poiseuille {l}x{m}x{n} {storage} rounding {rnd}, {steps} steps")
+        info, u = run_channel(storage, rnd, case, steps)
+        out[storage] = (info, u)
+    ref = out["f64"][1]
+    den = np.sqrt(sum((r ** 2).sum() for r in ref))
+    u16 = out["f16"][1]
+    l2_16 = np.sqrt(sum(((a - b) ** 2).sum() for a, b in zip(u16, ref))) / den
+    prof_err = {}
+    for storage, (info, u) in out.items():
+        prof = u[0].mean(axis=(0, 2))  # fields are (k, j, i): average over x and z
+        prof_err[storage] = float(np.sqrt(((prof - exact) ** 2).sum() / (exact ** 2).sum()))
+    print(f"[gpu] poiseuille: f16-SR L2 vs f64 {l2_16:.3e}; profile vs analytic {prof_err}; "
+          f"MLUPS f64 {out['f64'][0]['mlups']:.0f} f16 {out['f16'][0]['mlups']:.0f}", flush=True)
+    assert l2_16 <= 1e-3
+    assert prof_err["f64"] < 1e-4
+    assert prof_err["f16"] < 5e-4
