@@ -181,3 +181,33 @@ def test_engine_reproduces_reference_mixed_run(gpu, key):
     assert np.array_equal(I.astype(np.float64), P)
     assert hashlib.sha256(np.ascontiguousarray(I).tobytes()).hexdigest() == spec["sha256_interior"]
     O.assert_diagno_close(diags, spec["texts"]["diagno.dat"])
+
+
+@pytest.mark.parametrize("key", ["ldc_24_m1", "tgv_32_m1"])
+def test_lean_16bit_forms_bounded_by_reference_mixed1(gpu, key):
+    """The FP16 forms the bench measures (lean collide; nearest and
+    stochastic rounding) cannot be bit-compared with the reference: they
+    rearrange the FP32 arithmetic.  They are bounded here against the
+    reference-order FP16 form (mixed = 1), which the test above pins bit for
+    bit to the reference's MIXED1 build: the relative L2 distance of the
+    velocity fields after the fixture's steps stays within the FP16 storage
+    error (2^-11 per store, accumulated over the run): measured 2.8e-4 /
+    3.2e-4 (cavity, nearest / stochastic) and 4.1e-4 / 5.0e-4 (Taylor-Green),
+    bound 2e-3."""
+    spec = FIX["mixed"][key]
+    case = case_of(spec)
+    banner(f"lean FP16 forms against MIXED1 {key}")
+
+    def velocity(**kw):
+        with Engine(case, "f16", **kw) as eng:
+            eng.step(spec["steps"])
+            _, ux, uy, uz = eng.fields()
+        return np.stack([ux, uy, uz])
+
+    ref = velocity(mixed=1, rounding=0)
+    den = np.sqrt((ref ** 2).sum())
+    err = {}
+    for name, kw in (("lean_rn", dict(rounding=0)), ("lean_sr", dict(rounding=1, seed=5))):
+        err[name] = float(np.sqrt(((velocity(**kw) - ref) ** 2).sum()) / den)
+    print(f"[gpu] {key}: relative L2 velocity distance to the MIXED1 form {err}", flush=True)
+    assert max(err.values()) < 2e-3
