@@ -354,7 +354,12 @@ __device__ __forceinline__ void collide_store(const FusedArgs<S>& p, typename St
         // partly interior line (n = 134 ... 182) ran 18-24 % slower.
         bgk_collide_stream(x, p.omega, fx, fy, fz, p.cst, [&](auto dc, C v) {
             constexpr int d = decltype(dc)::value;
-            store_nt((d < 18 ? p.b[d] : p.a[18]) + (row + i), Rounder<S, 0>::cvt(v, 0u));
+            S* dst;
+            if constexpr (d < 18)
+                dst = p.b[d];
+            else
+                dst = p.a[18];
+            store_nt(dst + (row + i), Rounder<S, 0>::cvt(v, 0u));
             if constexpr (d == 18) flag_nonfinite(p, v);
         });
         return;

@@ -286,12 +286,12 @@ void one_step(bgk_handle& h, hipEvent_t pre, hipEvent_t post)
         hipLaunchKernelGGL(outflow_halo<S>, dim3(ceil_div((long)h.cfg.m * h.cfg.n, 256)), dim3(256), 0, h.stream,
                            h.current<S>(), h.geom());
     }
-    if (pre) hipEventRecord(pre, h.stream);
+    if (pre) (void)hipEventRecord(pre, h.stream);
     if (h.cfg.rounding && sizeof(S) == 2)
         dispatch_fused<S, 1>(h);
     else
         dispatch_fused<S, 0>(h);
-    if (post) hipEventRecord(post, h.stream);
+    if (post) (void)hipEventRecord(post, h.stream);
     h.cur = 1 - h.cur;
     ++h.steps;
 }
@@ -470,7 +470,8 @@ int bgk_create(const bgk_config* cfg, bgk_handle** out)
     if (hipMalloc(&region, lattice_bytes(*h)) != hipSuccess)
         return cleanup(fail(BGK_ENOMEM, "cannot allocate %.2f GB of lattice", lattice_bytes(*h) / 1e9));
     place_lattices(*h, region);
-    hipMemsetAsync(region, 0, lattice_bytes(*h), h->stream);
+    if (hipMemsetAsync(region, 0, lattice_bytes(*h), h->stream) != hipSuccess)
+        return cleanup(fail(BGK_EHIP, "lattice clear failed"));
 
     // obstacle: reference sphere shell by default (SRC/build.obs.F90:52-60)
     if (c.obstacle) {
@@ -539,15 +540,15 @@ int bgk_create(const bgk_config* cfg, bgk_handle** out)
 int bgk_destroy(bgk_handle* h)
 {
     if (!h) return 0;
-    if (h->stream) hipStreamSynchronize(h->stream);
-    for (auto e : h->events) hipEventDestroy(e);
-    if (h->obits) hipFree(h->obits);
-    if (h->omask) hipFree(h->omask);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto e : h->events) (void)hipEventDestroy(e);
+    if (h->obits) (void)hipFree(h->obits);
+    if (h->omask) (void)hipFree(h->omask);
     if (!h->borrowed) {
-        if (h->mem[0]) hipFree(h->mem[0]);  // one region holds both lattices
-        if (h->scratch) hipFree(h->scratch);
-        if (h->hscratch) hipHostFree(h->hscratch);
-        if (h->stream) hipStreamDestroy(h->stream);
+        if (h->mem[0]) (void)hipFree(h->mem[0]);  // one region holds both lattices
+        if (h->scratch) (void)hipFree(h->scratch);
+        if (h->hscratch) (void)hipHostFree(h->hscratch);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
     }
     delete h;
     return 0;
@@ -682,7 +683,7 @@ int bgk_fields(bgk_handle* h, double* rho, double* ux, double* uy, double* uz)
                                       h->stream) != hipSuccess)
             rc = fail(BGK_EHIP, "field download");
     if (hipStreamSynchronize(h->stream) != hipSuccess && rc == 0) rc = fail(BGK_EHIP, "fields");
-    hipFree(d);
+    (void)hipFree(d);
     return rc;
 }
 
@@ -758,7 +759,7 @@ int bgk_get_population(bgk_handle* h, int dir, double* out)
     if (hipMemcpyAsync(out, d, cells * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         rc = fail(BGK_EHIP, "population download");
-    hipFree(d);
+    (void)hipFree(d);
     return rc;
 }
 
@@ -780,7 +781,7 @@ int bgk_set_population(bgk_handle* h, int dir, const double* in)
             return 0;
         });
     if (hipStreamSynchronize(h->stream) != hipSuccess && rc == 0) rc = fail(BGK_EHIP, "population set");
-    hipFree(d);
+    (void)hipFree(d);
     return rc;
 }
 

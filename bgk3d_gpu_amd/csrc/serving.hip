@@ -242,8 +242,8 @@ namespace {
 int grow_group(bgks_server& s, bgks_server::Group& g, size_t slot_bytes, size_t owners)
 {
     if (slot_bytes > g.slot_bytes_cap) {
-        if (g.d_slots) hipFree(g.d_slots);
-        if (g.h_slots) hipHostFree(g.h_slots);
+        if (g.d_slots) (void)hipFree(g.d_slots);
+        if (g.h_slots) (void)hipHostFree(g.h_slots);
         g.d_slots = g.h_slots = nullptr;
         const size_t cap = std::max(slot_bytes, 2 * g.slot_bytes_cap);
         ENG_HIP(hipMalloc(&g.d_slots, cap));
@@ -251,8 +251,8 @@ int grow_group(bgks_server& s, bgks_server::Group& g, size_t slot_bytes, size_t 
         g.slot_bytes_cap = cap;
     }
     if (owners > g.owner_cap) {
-        if (g.d_owner) hipFree(g.d_owner);
-        if (g.h_owner) hipHostFree(g.h_owner);
+        if (g.d_owner) (void)hipFree(g.d_owner);
+        if (g.h_owner) (void)hipHostFree(g.h_owner);
         g.d_owner = g.h_owner = nullptr;
         const size_t cap = std::max(owners, 2 * g.owner_cap);
         ENG_HIP(hipMalloc(&g.d_owner, cap * sizeof(uint32_t)));
@@ -544,20 +544,20 @@ int bgks_set_policy(bgks_server* s, int policy, long long min_cells_per_step)
 int bgks_destroy(bgks_server* s)
 {
     if (!s) return 0;
-    hipSetDevice(s->device);
-    if (s->stream) hipStreamSynchronize(s->stream);
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto& g : s->groups) {
-        if (g.d_slots) hipFree(g.d_slots);
-        if (g.d_owner) hipFree(g.d_owner);
-        if (g.h_slots) hipHostFree(g.h_slots);
-        if (g.h_owner) hipHostFree(g.h_owner);
+        if (g.d_slots) (void)hipFree(g.d_slots);
+        if (g.d_owner) (void)hipFree(g.d_owner);
+        if (g.h_slots) (void)hipHostFree(g.h_slots);
+        if (g.h_owner) (void)hipHostFree(g.h_owner);
     }
-    if (s->arena_mem) hipFree(s->arena_mem);
-    if (s->scratch) hipFree(s->scratch);
-    if (s->hscratch) hipHostFree(s->hscratch);
-    if (s->d_flags) hipFree(s->d_flags);
-    if (s->h_flags) hipHostFree(s->h_flags);
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->arena_mem) (void)hipFree(s->arena_mem);
+    if (s->scratch) (void)hipFree(s->scratch);
+    if (s->hscratch) (void)hipHostFree(s->hscratch);
+    if (s->d_flags) (void)hipFree(s->d_flags);
+    if (s->h_flags) (void)hipHostFree(s->h_flags);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return 0;
 }
