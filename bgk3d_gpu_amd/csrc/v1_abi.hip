@@ -109,6 +109,13 @@ void launch_face(K kern, long count, const char* name, Args... args)
     report_launch(name);
 }
 
+// collide_flat uses no LDS; reserving this much per 256-thread block holds
+// it to 5 blocks (5 waves per SIMD) of a CU's 160 KiB.  Over the caller's 37
+// separately strided arrays fewer streams in flight ran faster: 496-498
+// against 510-511 us per FP32 launch at 256^3 (8 waves per SIMD); 6 waves
+// per SIMD gained half of that, 4 the same as 5.
+constexpr unsigned kFlatLdsCap = 32u << 10;
+
 // The fused kernel in halo mode: every pull reads the caller's halo, which
 // the bcond_*_gpu entry points filled.  Waves are aligned to 128-byte lines
 // from the output addresses, whatever padding (ipad) the caller chose.
@@ -156,9 +163,9 @@ void launch_col(const Lattice19<real_t>& A, const Lattice19<real_t>& B, const Ge
     if (g.nxy < (1L << 31) && ceil_div(flat_cells, 64L) < row_waves) {
         const dim3 grid((unsigned)ceil_div(flat_cells, 256L), (unsigned)g.n);
         if (g.nx % p.align == 0 && g.nxy % p.align == 0)
-            hipLaunchKernelGGL((collide_flat<real_t, FM, true>), grid, dim3(256), 0, abi_stream(), p);
+            hipLaunchKernelGGL((collide_flat<real_t, FM, true>), grid, dim3(256), kFlatLdsCap, abi_stream(), p);
         else
-            hipLaunchKernelGGL((collide_flat<real_t, FM, false>), grid, dim3(256), 0, abi_stream(), p);
+            hipLaunchKernelGGL((collide_flat<real_t, FM, false>), grid, dim3(256), kFlatLdsCap, abi_stream(), p);
         report_launch("collide_flat");
         return;
     }
