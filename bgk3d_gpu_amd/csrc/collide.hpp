@@ -310,33 +310,16 @@ inline RowMap row_map_capped(int span, int maxw)
     return r;
 }
 
-// Zeros into population d..18 of one cell that no boundary reads (row
-// padding, a halo cell of a folded boundary), so that its line is written
-// whole.
-template <typename S, bool CMP>
-__device__ __forceinline__ void store_zero_cell(const FusedArgs<S>& p, long row, int i)
-{
-    typedef typename Storage<S>::compute C;
-    constexpr long ES = (long)sizeof(S);
-#pragma unroll
-    for (int d = 0; d < 19; ++d) {
-        S* dst;
-        if constexpr (sizeof(S) >= 4)
-            dst = (d < 18 ? p.b[d] : p.a[18]) + (row + i);
-        else
-            dst = reinterpret_cast<S*>(reinterpret_cast<char*>(pop_b<CMP>(p, d)) + row * ES +
-                                       (uint64_t)((uint32_t)i * (uint32_t)ES));
-        store_nt(dst, Rounder<S, 0>::cvt(C(0), 0u));
-    }
-}
-
 // Collision and stores of one cell whose pulled populations are x.  `cell`
 // keys the stochastic-rounding stream: the cell's index in the reference's
 // (0:l+1, 0:m+1, 0:n+1) layout, so the stream does not depend on the
-// wave mapping or the row padding.
+// wave mapping or the row padding.  A lane with live = false (a padding or
+// halo position that only completes a 128-byte line) stores zeros through
+// the same store instructions, so a wave that covers both kinds of lanes
+// issues its 19 stores once instead of branching into a second set.
 template <typename S, int FMODE, int SR, bool CMP>
 __device__ __forceinline__ void collide_store(const FusedArgs<S>& p, typename Storage<S>::compute (&x)[19], int i,
-                                              long row, uint32_t seed, uint32_t cell)
+                                              long row, uint32_t seed, uint32_t cell, bool live = true)
 {
     typedef typename Storage<S>::compute C;
     constexpr long ES = (long)sizeof(S);
@@ -359,8 +342,9 @@ __device__ __forceinline__ void collide_store(const FusedArgs<S>& p, typename St
                 dst = p.b[d];
             else
                 dst = p.a[18];
-            store_nt(dst + (row + i), Rounder<S, 0>::cvt(v, 0u));
-            if constexpr (d == 18) flag_nonfinite(p, v);
+            const C w = live ? v : C(0);
+            store_nt(dst + (row + i), Rounder<S, 0>::cvt(w, 0u));
+            if constexpr (d == 18) flag_nonfinite(p, w);
         });
         return;
     }
@@ -381,9 +365,9 @@ __device__ __forceinline__ void collide_store(const FusedArgs<S>& p, typename St
             dst = (d < 18 ? p.b[d] : p.a[18]) + (row + i);
         else
             dst = reinterpret_cast<S*>(reinterpret_cast<char*>(pop_b<CMP>(p, d)) + row * ES + (uint64_t)vi);
-        store_nt(dst, Rounder<S, SR>::cvt(o[d], SR ? sr_word(r, d) : 0u));
+        store_nt(dst, Rounder<S, SR>::cvt(live ? o[d] : C(0), SR ? sr_word(r, d) : 0u));
     }
-    flag_nonfinite(p, o[18]);
+    flag_nonfinite(p, live ? o[18] : C(0));
 }
 
 // Index of cell (i, j, k) in the reference's (0:l+1, 0:m+1, 0:n+1) layout.
@@ -395,7 +379,11 @@ __device__ __forceinline__ uint32_t ref_cell(const FusedArgs<S>& p, int i, int j
 
 // The work of one block (bx, by, bz) of the launch geometry `rm` describes:
 // pull, collide, store.  `seed` is the step's stochastic-rounding seed.
-template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
+// PADZ: rows may end in lanes l < i <= lend that store zeros (below); with
+// PADZ false the launcher guarantees lend <= l, and the 16-bit forms, which
+// are sensitive to instruction issue, skip the per-store selects (their
+// 256^3 collide took 228 instead of 214 us with them).
+template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR, bool PADZ = true>
 __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& rm, int bx, int by, int bz,
                                             uint32_t seed)
 {
@@ -425,23 +413,24 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
     const int i = 1 - shift + seg * 64 + lane;
     if (i < 1) return;
     const long row = (long)p.nx * j + p.nxy * (long)k;  // scalar
-    if (i > p.l) {
-        // Whole-line stores: when the row's last 128-byte line is only partly
-        // interior, the lanes past i = l fill it (row padding and the next
-        // row's i = 0 halo cell, which a folded x boundary never reads), so
-        // HBM sees full-line writes instead of partial ones.  Only with a
-        // layout whose rows start on a line (lend set by the engine).
-        if (i <= p.lend) store_zero_cell<S, CMP>(p, row, i);
-        return;
-    }
+    // Whole-line stores: when the row's last 128-byte line is only partly
+    // interior, the lanes past i = l fill it with zeros (row padding and the
+    // next row's i = 0 halo cell, which a folded x boundary never reads), so
+    // HBM sees full-line writes instead of partial ones.  Only with a layout
+    // whose rows start on a line (lend set by the engine).  Those lanes run
+    // the pull on cell l (in bounds, lines the wave reads anyway) and store
+    // through the wave's own store instructions.
+    if (i > p.l && (!PADZ || i > p.lend)) return;
+    const bool live = !PADZ || i <= p.l;
+    const int il = live ? i : p.l;
 
     C x[19];
     const bool edge = (BY != AX_HALO && (j == 1 || j == p.m)) || (BZ != AX_HALO && (k == 1 || k == p.n));
     if (edge)
-        pull_all<S, BX, BY, BZ, OBST, true, CMP>(p, i, j, k, row, x);
+        pull_all<S, BX, BY, BZ, OBST, true, CMP>(p, il, j, k, row, x);
     else
-        pull_all<S, BX, BY, BZ, OBST, false, CMP>(p, i, j, k, row, x);
-    collide_store<S, FMODE, SR, CMP>(p, x, i, row, seed, ref_cell(p, i, j, k));
+        pull_all<S, BX, BY, BZ, OBST, false, CMP>(p, il, j, k, row, x);
+    collide_store<S, FMODE, SR, CMP>(p, x, i, row, seed, ref_cell(p, il, j, k), live);
 }
 
 // The compact layout of the handle API: rows of l + 2 cells without padding
@@ -468,13 +457,16 @@ __device__ __forceinline__ void compact_cells(const FusedArgs<S>& p, int bx, int
     const int j = 1 + r / p.nx;
     const int i = r - (j - 1) * p.nx;
     const long row = (long)p.nx * j + p.nxy * (long)k;
-    if (j > p.m || i == 0 || i > p.l) {
-        store_zero_cell<S, CMP>(p, row, i);
-        return;
-    }
+    // halo positions (i = 0, i = l + 1, row m + 1) store zeros through the
+    // wave's own store instructions after a pull on the nearest interior
+    // cell (collide_store, live = false)
+    const bool live = j <= p.m && i >= 1 && i <= p.l;
+    const int il = i < 1 ? 1 : (i > p.l ? p.l : i);
+    const int jl = j > p.m ? p.m : j;
+    const long rowl = (long)p.nx * jl + p.nxy * (long)k;
     C x[19];
-    pull_all<S, BX, BY, BZ, OBST, true, CMP>(p, i, j, k, row, x);
-    collide_store<S, FMODE, SR, CMP>(p, x, i, row, seed, ref_cell(p, i, j, k));
+    pull_all<S, BX, BY, BZ, OBST, true, CMP>(p, il, jl, k, rowl, x);
+    collide_store<S, FMODE, SR, CMP>(p, x, i, row, seed, ref_cell(p, il, jl, k), live);
 }
 
 // Waves per SIMD the compiler is asked to fit (1: no request).  FP64
@@ -488,11 +480,24 @@ struct FusedWaves {
     static constexpr int value = (!LES && std::is_same<typename Storage<S>::compute, double>::value) ? 6 : 1;
 };
 
-template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
+template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR, bool PADZ = true>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FusedWaves<S, FMODE>::value))) void collide_fused(
     FusedArgs<S> p, RowMap rm)
 {
-    fused_rows<S, BX, BY, BZ, OBST, FMODE, SR>(p, rm, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, p.seed);
+    fused_rows<S, BX, BY, BZ, OBST, FMODE, SR, PADZ>(p, rm, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
+                                                      p.seed);
+}
+
+// Dynamic LDS the launchers reserve per collide_compact block.  The kernel
+// uses none; 32 KiB per 256-thread block holds the FP32 form to 5 waves per
+// SIMD (a CU has 160 KiB) instead of 8: fewer streams in flight over rows
+// that do not start on a line ran 8 % faster at 256^3 (463 against 502 us)
+// and 4-5 % at n = 129 ... 192.  The 16-bit forms ran slower so capped (259
+// against 245 us at 256^3) and keep their natural occupancy.
+template <typename S>
+constexpr unsigned compact_lds_cap()
+{
+    return std::is_same<S, float>::value ? (32u << 10) : 0u;
 }
 
 template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>

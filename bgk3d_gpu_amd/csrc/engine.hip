@@ -224,11 +224,19 @@ void launch_fused(bgk_handle& h)
     if constexpr (BX != AX_HALO) {
         if (h.compact) {
             hipLaunchKernelGGL((collide_compact<S, BX, BY, BZ, OB, FM, SR>),
-                               dim3((unsigned)ceil_div(p.lend, 256), (unsigned)p.n), dim3(256), 0, h.stream, p);
+                               dim3((unsigned)ceil_div(p.lend, 256), (unsigned)p.n), dim3(256), compact_lds_cap<S>(),
+                               h.stream, p);
             return;
         }
     }
     const RowMap rm = row_map(p.l);
+    if constexpr (sizeof(S) == 2) {
+        if (p.lend <= p.l) {
+            hipLaunchKernelGGL((collide_fused<S, BX, BY, BZ, OB, FM, SR, false>), fused_grid(rm, p.m, p.n),
+                               fused_block(rm), 0, h.stream, p, rm);
+            return;
+        }
+    }
     hipLaunchKernelGGL((collide_fused<S, BX, BY, BZ, OB, FM, SR>), fused_grid(rm, p.m, p.n), fused_block(rm), 0,
                        h.stream, p, rm);
 }
@@ -374,9 +382,11 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
     h.ny = c.m + 2;
     h.nz = c.n + 2;
     // Rows padded to whole lines, unless the last line of a row would hold
-    // more than kCompactWaste of a row in unused cells: then compact rows.
+    // more than compact_waste_threshold of a row in unused cells: then
+    // compact rows.
     const double waste = (double)(round_up(c.l, h.A) - c.l) / (double)c.l;
-    h.compact = c.boundary != BGK_BC_INFLOW && (c.layout == 2 || (c.layout == 0 && waste > kCompactWaste));
+    h.compact = c.boundary != BGK_BC_INFLOW &&
+                (c.layout == 2 || (c.layout == 0 && waste > compact_waste_threshold(c)));
     if (h.compact) {
         // cell (0, 1, k) of every population plane on a line start
         h.nx = c.l + 2;

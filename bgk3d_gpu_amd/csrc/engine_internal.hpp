@@ -105,11 +105,16 @@ inline size_t storage_bytes(int kind)
 inline long round_up(long v, long a) { return (v + a - 1) / a * a; }
 
 // Compact rows are chosen when padded rows would leave more than this share
-// of a row's cells unused in its last line.  Measured (cavities alone, FP32):
-// compact rows run 14-18 % slower than padded ones where a row fills its
-// lines (the y-shifted pulls then span three lines per wave), and 4-17 %
-// faster at n = 97, 129, 136, 161 (waste 18-32 %); FP16 likewise.
-constexpr double kCompactWaste = 0.12;
+// of a row's cells unused in its last line.  Measured (cavities alone, one
+// handle): where a row fills its lines, padded rows are faster (FP32 at 256^3
+// 438 against 463 us, FP16 214 against 245 us); with FP32 or 16-bit storage,
+// compact rows are as fast at 2 % waste (n = 188, 250) and 3-21 % faster
+// from 3.5 % up (n = 100 ... 230).  FP64 compact rows ran 6-24 % slower than
+// padded ones at most sizes up to 11 % waste, hence its higher threshold.
+inline double compact_waste_threshold(const bgk_config& c)
+{
+    return (c.storage == BGK_F64 || c.mixed == 2) ? 0.12 : 0.03;
+}
 
 // Calls f with a value of the storage type the configuration selects: the
 // storage kind, or for the mixed forms the type that carries their

@@ -329,7 +329,12 @@ void launch_group(bgks_server::Group& g, const SubBatch& sb, int t, hipStream_t 
         p.seed = step_seed(sl.cfg_seed, sl.steps0 + t);
         if (compact) {
             hipLaunchKernelGGL((collide_compact<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>),
-                               dim3((unsigned)ceil_div(p.lend, 256), (unsigned)p.n), dim3(256), 0, stream, p);
+                               dim3((unsigned)ceil_div(p.lend, 256), (unsigned)p.n), dim3(256), compact_lds_cap<S>(),
+                               stream, p);
+        } else if (sizeof(S) == 2 && p.lend <= p.l) {
+            const RowMap rm = row_map(p.l);
+            hipLaunchKernelGGL((collide_fused<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR, sizeof(S) != 2>),
+                               fused_grid(rm, p.m, p.n), fused_block(rm), 0, stream, p, rm);
         } else {
             const RowMap rm = row_map(p.l);
             hipLaunchKernelGGL((collide_fused<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>), fused_grid(rm, p.m, p.n),
@@ -338,7 +343,7 @@ void launch_group(bgks_server::Group& g, const SubBatch& sb, int t, hipStream_t 
         return;
     }
     if (compact)
-        hipLaunchKernelGGL((collide_batched_compact<S, SR>), dim3(sb.nblocks), dim3(256), 0, stream,
+        hipLaunchKernelGGL((collide_batched_compact<S, SR>), dim3(sb.nblocks), dim3(256), compact_lds_cap<S>(), stream,
                            static_cast<const BatchSlot<S>*>(g.d_slots), g.d_owner, sb.block0, t);
     else
         hipLaunchKernelGGL((collide_batched<S, SR>), dim3(sb.nblocks), dim3(64 * kServeWaves), 0, stream,
