@@ -382,11 +382,9 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
     h.ny = c.m + 2;
     h.nz = c.n + 2;
     // Rows padded to whole lines, unless the last line of a row would hold
-    // more than compact_waste_threshold of a row in unused cells: then
-    // compact rows.
+    // more than kCompactWaste of a row in unused cells: then compact rows.
     const double waste = (double)(round_up(c.l, h.A) - c.l) / (double)c.l;
-    h.compact = c.boundary != BGK_BC_INFLOW &&
-                (c.layout == 2 || (c.layout == 0 && waste > compact_waste_threshold(c)));
+    h.compact = c.boundary != BGK_BC_INFLOW && (c.layout == 2 || (c.layout == 0 && waste > kCompactWaste));
     if (h.compact) {
         // cell (0, 1, k) of every population plane on a line start
         h.nx = c.l + 2;

@@ -107,14 +107,11 @@ inline long round_up(long v, long a) { return (v + a - 1) / a * a; }
 // Compact rows are chosen when padded rows would leave more than this share
 // of a row's cells unused in its last line.  Measured (cavities alone, one
 // handle): where a row fills its lines, padded rows are faster (FP32 at 256^3
-// 438 against 463 us, FP16 214 against 245 us); with FP32 or 16-bit storage,
-// compact rows are as fast at 2 % waste (n = 188, 250) and 3-21 % faster
-// from 3.5 % up (n = 100 ... 230).  FP64 compact rows ran 6-24 % slower than
-// padded ones at most sizes up to 11 % waste, hence its higher threshold.
-inline double compact_waste_threshold(const bgk_config& c)
-{
-    return (c.storage == BGK_F64 || c.mixed == 2) ? 0.12 : 0.03;
-}
+// 438 against 463 us, FP16 214 against 245 us, FP64 at 112^3 74.8 against
+// 78.0 us); at 2 % waste the two are even (compact/padded 0.98-1.02 at
+// n = 188, 250 in FP32, FP16 and FP64), and from 3.5 % up compact rows are
+// 2-21 % faster (n = 100 ... 230).
+constexpr double kCompactWaste = 0.03;
 
 // Calls f with a value of the storage type the configuration selects: the
 // storage kind, or for the mixed forms the type that carries their

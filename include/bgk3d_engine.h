@@ -66,9 +66,8 @@ typedef struct bgk_config {
                              FP32 arithmetic); 2 MIXED2: storage BGK_F32, collision in FP64 */
     int layout;           /* 0 choose, 1 rows padded to 128-byte lines, 2 compact rows of l + 2
                              cells (not with BGK_BC_INFLOW).  The default pads the rows unless
-                             that leaves more than 3 % of a row unused in its last line (12 % for
-                             FP64 and MIXED2; a line holds 32 values in FP32, 16 in FP64, 64 in
-                             16-bit storage) */
+                             that leaves more than 3 % of a row unused in its last line (a line
+                             holds 32 values in FP32, 16 in FP64, 64 in 16-bit storage) */
 } bgk_config;
 
 typedef struct bgk_handle bgk_handle;

@@ -9,12 +9,12 @@ from bgk3d_gpu_amd import serving as S
 def layout_bytes(n, esize):
     # rows padded to 128-byte lines (start shifted so i = 1 opens a line),
     # unless the last line of a row would leave more than 3 % of a row
-    # unused (12 % for FP64): then compact rows of n + 2 cells with each
-    # plane padded to whole lines; 19 population planes per z level in each
-    # of the two lattices, request region rounded to 64 KiB
+    # unused: then compact rows of n + 2 cells with each plane padded to
+    # whole lines; 19 population planes per z level in each of the two
+    # lattices, request region rounded to 64 KiB
     A = 128 // esize
     up = lambda v: -(-v // A) * A  # noqa: E731
-    if (up(n) - n) / n > (0.12 if esize == 8 else 0.03):
+    if (up(n) - n) / n > 0.03:
         plane = up((n + 2) * (n + 2))
     else:
         plane = up(n + 2) * (n + 2)
