@@ -148,6 +148,15 @@ __device__ __forceinline__ typename Storage<S>::compute pull_load(const S* ptr)
     else
         return Storage<S>::load(ptr);
 }
+// The same load without the widening (16-bit storage).
+template <typename S, int CX>
+__device__ __forceinline__ S pull_raw(const S* ptr)
+{
+    if constexpr (CX == 0)
+        return Storage<S>::raw_nt(ptr);
+    else
+        return Storage<S>::raw(ptr);
+}
 
 // The moving lid: a pull across the top wall adds +force to direction 2
 // and -force to direction 11 (d = 1 and 10 here).  The reference stores that
@@ -204,6 +213,8 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
         if (ri < (unsigned)p.oni && rj < (unsigned)p.onj && rk < (unsigned)p.onk)
             ob = p.obits[(long)ri + (long)p.oni * ((long)rj + (long)p.onj * rk)];
     }
+    [[maybe_unused]] S raw[19];      // 16-bit storage: the loaded values before widening
+    [[maybe_unused]] bool tops[19];  // 16-bit storage: pulls across the moving lid
     for_each_dir([&](auto dc) {
         constexpr int d = decltype(dc)::value;
         constexpr int cx = cx_of(d), cy = cy_of(d), cz = cz_of(d);
@@ -226,35 +237,47 @@ __device__ __forceinline__ void pull_all(const FusedArgs<S>& p, int i, int j, in
             if (BZ == AX_PERIODIC) ds += (long)sh * p.n * p.nxy;
         }
         if (OBST && d < 18) bounce = bounce || ((ob >> d) & 1u);
-        C v;
         if constexpr (sizeof(S) >= 4) {
             // 32/64-bit storage is bound by HBM, not by issue: per-lane 64-bit
             // addresses keep the scalar registers free (higher occupancy)
             const long s = row + i;
             const long src = s + ds + dx;
             const bool b = wall || bounce;
-            v = pull_load<S, NT0 ? cx : 1>(b ? (p.a[opp_of(d)] + s) : (p.a[d] + src));
+            C v = pull_load<S, NT0 ? cx : 1>(b ? (p.a[opp_of(d)] + s) : (p.a[d] + src));
             if (BZ == AX_LID && EDGE) v = lid_value<S, d>(v, top, p.lidforce);
             x[d] = v;
-            return;
-        }
-        const char* own = reinterpret_cast<const char*>(pop_a<CMP>(p, opp_of(d))) + rowb;
-        if (wall) {  // y or z wall: the whole wave bounces
-            v = pull_load<S, NT0 ? cx : 1>(reinterpret_cast<const S*>(own + (uint64_t)vi));
         } else {
-            const char* base = reinterpret_cast<const char*>(pop_a<CMP>(p, d)) + rowb + ds * ES;
-            const S* nptr = reinterpret_cast<const S*>(base + (uint64_t)(vi + (uint32_t)dx * (uint32_t)ES));
-            if ((BX == AX_WALL && cx != 0) || (OBST && d < 18)) {
-                // a lane may bounce: pick its address, one load
-                const S* optr = reinterpret_cast<const S*>(own + (uint64_t)vi);
-                v = pull_load<S, NT0 ? cx : 1>(bounce ? optr : nptr);
+            // 16-bit storage: every raw load is issued here and widened after
+            // the scheduling barrier below, so the 19 loads are in flight
+            // together (left to itself the scheduler has reused one register
+            // for all of them and waited for each load in turn, which doubled
+            // the time of some boundary forms)
+            const char* own = reinterpret_cast<const char*>(pop_a<CMP>(p, opp_of(d))) + rowb;
+            if (wall) {  // y or z wall: the whole wave bounces
+                raw[d] = pull_raw<S, NT0 ? cx : 1>(reinterpret_cast<const S*>(own + (uint64_t)vi));
             } else {
-                v = pull_load<S, NT0 ? cx : 1>(nptr);
+                const char* base = reinterpret_cast<const char*>(pop_a<CMP>(p, d)) + rowb + ds * ES;
+                const S* nptr = reinterpret_cast<const S*>(base + (uint64_t)(vi + (uint32_t)dx * (uint32_t)ES));
+                if ((BX == AX_WALL && cx != 0) || (OBST && d < 18)) {
+                    // a lane may bounce: pick its address, one load
+                    const S* optr = reinterpret_cast<const S*>(own + (uint64_t)vi);
+                    raw[d] = pull_raw<S, NT0 ? cx : 1>(bounce ? optr : nptr);
+                } else {
+                    raw[d] = pull_raw<S, NT0 ? cx : 1>(nptr);
+                }
             }
+            tops[d] = top;
         }
-        if (BZ == AX_LID && EDGE) v = lid_value<S, d>(v, top, p.lidforce);
-        x[d] = v;
     });
+    if constexpr (sizeof(S) == 2) {
+        __builtin_amdgcn_sched_barrier(0);
+        for_each_dir([&](auto dc) {
+            constexpr int d = decltype(dc)::value;
+            C v = Storage<S>::widen(raw[d]);
+            if (BZ == AX_LID && EDGE) v = lid_value<S, d>(v, tops[d], p.lidforce);
+            x[d] = v;
+        });
+    }
 }
 
 // Work mapping.  wpr = waves needed per (possibly shifted) row.  A block

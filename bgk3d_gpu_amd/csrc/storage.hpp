@@ -88,6 +88,8 @@ struct Storage<f16_t> {
     __device__ __forceinline__ static float load(const f16_t* p) { return (float)*gmem(p); }
     __device__ __forceinline__ static float load_nt(const f16_t* p) { return (float)__builtin_nontemporal_load(gmem(p)); }
     __device__ __forceinline__ static float widen(f16_t v) { return (float)v; }
+    __device__ __forceinline__ static f16_t raw(const f16_t* p) { return *gmem(p); }
+    __device__ __forceinline__ static f16_t raw_nt(const f16_t* p) { return __builtin_nontemporal_load(gmem(p)); }
 };
 template <>
 struct Storage<bf16_t> {
@@ -103,6 +105,11 @@ struct Storage<bf16_t> {
         return __uint_as_float((uint32_t)__builtin_nontemporal_load(&gmem(p)->u) << 16);
     }
     __device__ __forceinline__ static float widen(bf16_t v) { return __uint_as_float((uint32_t)v.u << 16); }
+    __device__ __forceinline__ static bf16_t raw(const bf16_t* p) { return bf16_t{gmem(p)->u}; }
+    __device__ __forceinline__ static bf16_t raw_nt(const bf16_t* p)
+    {
+        return bf16_t{__builtin_nontemporal_load(&gmem(p)->u)};
+    }
 };
 template <>
 struct Storage<mix32_t> {
@@ -127,6 +134,11 @@ struct Storage<f16r_t> {
         return (float)__builtin_nontemporal_load(&gmem(p)->v);
     }
     __device__ __forceinline__ static float widen(f16r_t v) { return (float)v.v; }
+    __device__ __forceinline__ static f16r_t raw(const f16r_t* p) { return f16r_t{gmem(p)->v}; }
+    __device__ __forceinline__ static f16r_t raw_nt(const f16r_t* p)
+    {
+        return f16r_t{__builtin_nontemporal_load(&gmem(p)->v)};
+    }
 };
 
 __host__ __device__ inline uint32_t hash32(uint32_t x)
