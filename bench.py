@@ -124,15 +124,16 @@ def cpu_baseline(case, steps):
             "sample": f"C oracle, OpenMP {threads} threads, Taylor-Green {case.l}^3 FP32, {steps} steps"}
 
 
-def load_traffic(workload):
+def load_traffic(workload, kernel):
     """HBM bytes per launch of the collide from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json, made by the
-    command in profiles/collide_fused_taylor_green_256_fp32.md); not measured
-    in this run."""
+    FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json, keyed by workload
+    and kernel, made by the command in the kernel's profile under
+    profiles/); not measured in this run.  None when no pass of this kernel
+    on this workload is recorded."""
     p = REPO / "profiles" / "traffic.json"
     if p.exists():
         data = json.loads(p.read_text())
-        return data.get(workload)
+        return data.get(f"{workload}:{kernel}")
     return None
 
 
@@ -348,7 +349,9 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(case, args.cpu_steps)
         log(f"cpu baseline {cpu['value']:.2f} MLUPS ({cpu['kind']}, {cpu['cores']} cores)")
-    traffic = load_traffic(workload)
+    compact = info["nx"] == n + 2  # the engine chose compact rows
+    kernel = "collide_compact" if compact else "collide_fused"
+    traffic = load_traffic(workload, kernel)
     out = {
         "metric": METRIC,
         "value": round(mlups, 1),
@@ -363,14 +366,15 @@ def main():
         "dtype": "f32" if args.prec == "sp" else "f64",
         "data": "synthetic (analytic Taylor-Green initial state, no dataset)",
         "config": {"workload": workload, "grid": [n, n, n], "boundary": "periodic", "tau": 0.53, "U0": 0.05,
-                   "path": "handle API: aligned layout, periodic boundary folded into the fused step",
+                   "path": "handle API: " + ("compact rows (nx = l + 2)" if compact else "rows padded to 128-byte "
+                                              "lines") + ", periodic boundary folded into the fused step",
                    "layout_nx": info["nx"], "device_bytes": info["bytes"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": "profiles/traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE passes of this "
-                                       "kernel (profiles/collide_fused_taylor_green_256_fp32.md), not this run",
+                     "traffic_source": f"profiles/traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE passes of this "
+                                       f"kernel (profiles/{kernel}_taylor_green_256_fp32.md), not this run",
                      "traffic_over_alg": round(traffic / alg_bytes, 4) if traffic else None,
-                     "kernel": "collide_fused", "kernel_avg_us": round(kern_avg_ms * 1e3, 2),
+                     "kernel": kernel, "kernel_avg_us": round(kern_avg_ms * 1e3, 2),
                      "alg_bytes_per_launch": alg_bytes,
                      "measured_copy_GBs": round(stream, 1) if stream else None,
                      "frac_of_measured_copy": round(achieved / stream, 4) if stream else None},

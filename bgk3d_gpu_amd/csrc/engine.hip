@@ -381,10 +381,11 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
     h.A = (int)(128 / h.esize);
     h.ny = c.m + 2;
     h.nz = c.n + 2;
-    // Rows padded to whole lines, unless the last line of a row would hold
-    // more than kCompactWaste of a row in unused cells: then compact rows.
+    // Rows padded to whole lines, or compact rows where those measured
+    // faster (prefer_compact: enough of a row's last line unused, or, with
+    // 32/64-bit storage, a lattice of 2^24 cells or more).
     const double waste = (double)(round_up(c.l, h.A) - c.l) / (double)c.l;
-    h.compact = c.boundary != BGK_BC_INFLOW && (c.layout == 2 || (c.layout == 0 && waste > kCompactWaste));
+    h.compact = c.boundary != BGK_BC_INFLOW && (c.layout == 2 || (c.layout == 0 && prefer_compact(c, waste)));
     if (h.compact) {
         // cell (0, 1, k) of every population plane on a line start
         h.nx = c.l + 2;

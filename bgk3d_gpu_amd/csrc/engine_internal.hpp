@@ -104,14 +104,20 @@ inline size_t storage_bytes(int kind)
 
 inline long round_up(long v, long a) { return (v + a - 1) / a * a; }
 
-// Compact rows are chosen when padded rows would leave more than this share
-// of a row's cells unused in its last line.  Measured (cavities alone, one
-// handle): where a row fills its lines, padded rows are faster (FP32 at 256^3
-// 438 against 463 us, FP16 214 against 245 us, FP64 at 112^3 74.8 against
-// 78.0 us); at 2 % waste the two are even (compact/padded 0.98-1.02 at
-// n = 188, 250 in FP32, FP16 and FP64), and from 3.5 % up compact rows are
-// 2-21 % faster (n = 100 ... 230).
-constexpr double kCompactWaste = 0.03;
+// Row layout of the automatic choice (layout = 0).  Measured kernel times,
+// compact/padded (cavities, Taylor-Green and channels alone, one handle):
+//  * FP32 and FP64: 0.94-0.99 from 3.5 % row waste up (n = 100 ... 250),
+//    even at 2 %; at zero waste 1.09-1.14 at 64^3, 1.00-1.04 from 96^3 to
+//    224^3, 0.94-0.99 from 256^3 up (384^3, 1024 x 384^2, 512 x 512 x 256).
+//  * 16-bit storage: 0.86-0.99 from 10 % waste up, even at 6-8 %, 1.04-1.07
+//    at 2-4 % (n = 184, 188, 250) and 1.06-1.13 at zero waste (256^3), 0.97
+//    at 384^3.
+inline bool prefer_compact(const bgk_config& c, double waste)
+{
+    const bool wide = storage_bytes(c.storage) >= 4;
+    if (waste > (wide ? 0.03 : 0.09)) return true;
+    return wide && (double)c.l * (double)c.m * (double)c.n >= (double)(1 << 24);
+}
 
 // Calls f with a value of the storage type the configuration selects: the
 // storage kind, or for the mixed forms the type that carries their

@@ -65,9 +65,11 @@ typedef struct bgk_config {
                              reference's expression order (the plain F16 form uses a leaner
                              FP32 arithmetic); 2 MIXED2: storage BGK_F32, collision in FP64 */
     int layout;           /* 0 choose, 1 rows padded to 128-byte lines, 2 compact rows of l + 2
-                             cells (not with BGK_BC_INFLOW).  The default pads the rows unless
-                             that leaves more than 3 % of a row unused in its last line (a line
-                             holds 32 values in FP32, 16 in FP64, 64 in 16-bit storage) */
+                             cells (not with BGK_BC_INFLOW).  The default takes compact rows
+                             where they measured faster: when padding would leave more than 3 %
+                             of a row unused in its last line (9 % with 16-bit storage; a line
+                             holds 32 values in FP32, 16 in FP64, 64 in 16-bit storage), and
+                             with FP32/FP64 storage for lattices of 2^24 cells or more */
 } bgk_config;
 
 typedef struct bgk_handle bgk_handle;

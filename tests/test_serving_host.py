@@ -9,12 +9,14 @@ from bgk3d_gpu_amd import serving as S
 def layout_bytes(n, esize):
     # rows padded to 128-byte lines (start shifted so i = 1 opens a line),
     # unless the last line of a row would leave more than 3 % of a row
-    # unused: then compact rows of n + 2 cells with each plane padded to
-    # whole lines; 19 population planes per z level in each of the two
-    # lattices, request region rounded to 64 KiB
+    # unused (9 % with 16-bit storage) or the lattice has 2^24 cells or more
+    # with 32/64-bit storage: then compact rows of n + 2 cells with each
+    # plane padded to whole lines; 19 population planes per z level in each
+    # of the two lattices, request region rounded to 64 KiB
     A = 128 // esize
     up = lambda v: -(-v // A) * A  # noqa: E731
-    if (up(n) - n) / n > 0.03:
+    wide = esize >= 4
+    if (up(n) - n) / n > (0.03 if wide else 0.09) or (wide and n ** 3 >= 1 << 24):
         plane = up((n + 2) * (n + 2))
     else:
         plane = up(n + 2) * (n + 2)
@@ -25,7 +27,8 @@ def layout_bytes(n, esize):
 
 @pytest.mark.parametrize("n,storage,esize", [(32, "f32", 4), (33, "f16", 2), (130, "f64", 8), (192, "bf16", 2),
                                              (129, "f32", 4), (180, "f32", 4), (97, "f16", 2), (188, "f32", 4),
-                                             (184, "f32", 4), (250, "f16", 2), (120, "f64", 8)])
+                                             (184, "f32", 4), (250, "f16", 2), (120, "f64", 8), (256, "f32", 4),
+                                             (256, "f16", 2), (200, "f16", 2), (184, "bf16", 2)])
 def test_request_bytes_follow_the_layout(n, storage, esize):
     assert S.request_bytes(S.Request(n, 1000.0, storage)) == layout_bytes(n, esize)
 
