@@ -331,15 +331,18 @@ void launch_group(bgks_server::Group& g, const SubBatch& sb, int t, hipStream_t 
             hipLaunchKernelGGL((collide_compact<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>),
                                dim3((unsigned)ceil_div(p.lend, 256), (unsigned)p.n), dim3(256), compact_lds_cap<S>(),
                                stream, p);
-        } else if (sizeof(S) == 2 && p.lend <= p.l) {
-            const RowMap rm = row_map(p.l);
-            hipLaunchKernelGGL((collide_fused<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR, sizeof(S) != 2>),
-                               fused_grid(rm, p.m, p.n), fused_block(rm), 0, stream, p, rm);
-        } else {
-            const RowMap rm = row_map(p.l);
-            hipLaunchKernelGGL((collide_fused<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>), fused_grid(rm, p.m, p.n),
-                               fused_block(rm), 0, stream, p, rm);
+            return;
         }
+        const RowMap rm = row_map(p.l);
+        if constexpr (sizeof(S) == 2) {
+            if (p.lend <= p.l) {  // no line-completing lanes: the form without the selects
+                hipLaunchKernelGGL((collide_fused<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR, false>),
+                                   fused_grid(rm, p.m, p.n), fused_block(rm), 0, stream, p, rm);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((collide_fused<S, AX_WALL, AX_WALL, AX_LID, false, 0, SR>), fused_grid(rm, p.m, p.n),
+                           fused_block(rm), 0, stream, p, rm);
         return;
     }
     if (compact)
