@@ -463,8 +463,9 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
 // consecutive cells per wave, line-aligned, up to the line end after
 // (l+1, m) (p.lend positions): lanes on interior cells pull, collide and
 // store; lanes on the x halo cells and on the cells of row m+1 store zeros,
-// so every line is written whole.  Only with folded boundaries, which never
-// read a halo cell.  The padded layout spends whole lines on a row's last
+// so every line is written whole.  A folded boundary never reads a halo
+// cell; the inflow boundary reads both x halos, and its outflow kernel
+// rewrites them in the source lattice before each collide.  The padded layout spends whole lines on a row's last
 // few cells (a 129-cell FP32 row spans five lines for 4.03 lines of data);
 // this one wastes 2 cells per row, with rows that start anywhere in a line
 // (y-shifted pulls then span three lines per wave, the third shared with the

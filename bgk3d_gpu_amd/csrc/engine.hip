@@ -61,18 +61,13 @@ namespace {
 // kernels specific to the engine
 // ---------------------------------------------------------------------------
 
-// Inflow halo x = 0, directions 1..5, every (j,k) incl. halo rows: the
-// shifted equilibrium for rho = 1, u = (u_inflow, 0, 0) of
-// SRC/bcond.inflow.F90:165-181 (cte1 = 0).  Constant in time, so it is
-// written once into both lattices.
+// Shifted equilibrium of the inflow face x = 0, directions 1..5, for
+// rho = 1, u = (u_inflow, 0, 0) (SRC/bcond.inflow.F90:165-181, cte1 = 0):
+// the same values for every (j, k).
 template <typename S>
-__global__ void inflow_halo(Lattice19<S> a, Geometry g, typename Storage<S>::compute u_inflow)
+__device__ __forceinline__ void inflow_values(typename Storage<S>::compute u_inflow, S (&v)[5])
 {
     typedef typename Storage<S>::compute R;
-    const int nj = g.m + 2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nj * (g.n + 2)) return;
-    const int j = t % nj, k = t / nj;
     const R p1 = W<R>::p1, p2 = W<R>::p2, rf = R(3), qf = R(1.5), three = R(3), crho = R(1), cte1 = R(0);
     const R xj = u_inflow, yj = R(0), zj = R(0);
     const R cvsq = xj * xj + yj * yj + zj * zj;
@@ -81,21 +76,39 @@ __global__ void inflow_halo(Lattice19<S> a, Geometry g, typename Storage<S>::com
     const R cx03 = rf * (xj + yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
     const R cx04 = rf * (xj + zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
     const R cx05 = rf * (xj) + qf * (three * (xj) * (xj)-cvsq);
+    v[0] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx01), 0u);
+    v[1] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx02), 0u);
+    v[2] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx03), 0u);
+    v[3] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx04), 0u);
+    v[4] = Rounder<S, 0>::cvt(crho * p1 * (cte1 + cx05), 0u);
+}
+
+// Inflow halo x = 0, directions 1..5, every (j,k) incl. halo rows.
+// Constant in time: with rows padded to whole lines the collide never
+// writes a halo cell, and this runs once per lattice.
+template <typename S>
+__global__ void inflow_halo(Lattice19<S> a, Geometry g, typename Storage<S>::compute u_inflow)
+{
+    const int nj = g.m + 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nj * (g.n + 2)) return;
+    const int j = t % nj, k = t / nj;
+    S v[5];
+    inflow_values<S>(u_inflow, v);
     const long s = gidx(g, 0, j, k);
-    store_value(a.p[0] + s, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx01), 0u));
-    store_value(a.p[1] + s, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx02), 0u));
-    store_value(a.p[2] + s, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx03), 0u));
-    store_value(a.p[3] + s, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx04), 0u));
-    store_value(a.p[4] + s, Rounder<S, 0>::cvt(crho * p1 * (cte1 + cx05), 0u));
+    for (int d = 0; d < 5; ++d) store_value(a.p[d] + s, v[d]);
 }
 
 // Outflow halo x = l+1, directions 10..14, interior (j,k): shifted
 // equilibrium with rho = 1 and the momentum of cell (l,j,k)
 // (SRC/bcond.inflow.F90:131-162).  The fused pull wraps j and k before
 // reading this halo, which reproduces the reference's y/z periodic copies of
-// the halo rows.
+// the halo rows.  With compact rows the previous collide stored zeros over
+// both x halos of this lattice (its lanes write whole lines), so `refill`
+// also writes the inflow face x = 0 of the rows the pull reads (interior j,
+// k), with the values inflow_halo writes.
 template <typename S>
-__global__ void outflow_halo(Lattice19<S> a, Geometry g)
+__global__ void outflow_halo(Lattice19<S> a, Geometry g, bool refill, typename Storage<S>::compute u_inflow)
 {
     typedef typename Storage<S>::compute R;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -122,6 +135,12 @@ __global__ void outflow_halo(Lattice19<S> a, Geometry g)
     store_value(a.p[11] + so, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx12), 0u));
     store_value(a.p[12] + so, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx13), 0u));
     store_value(a.p[13] + so, Rounder<S, 0>::cvt(crho * p1 * (cte1 + cx14), 0u));
+    if (refill) {
+        S v[5];
+        inflow_values<S>(u_inflow, v);
+        const long s0 = gidx(g, 0, j, k);
+        for (int d = 0; d < 5; ++d) store_value(a.p[d] + s0, v[d]);
+    }
 }
 
 // Momentum exchange over fluid cells of [is..ie]x[js..je]x[ks..ke]
@@ -221,13 +240,11 @@ template <typename S, int BX, int BY, int BZ, bool OB, int FM, int SR>
 void launch_fused(bgk_handle& h)
 {
     const FusedArgs<S> p = fused_args<S>(h, h.cur);
-    if constexpr (BX != AX_HALO) {
-        if (h.compact) {
-            hipLaunchKernelGGL((collide_compact<S, BX, BY, BZ, OB, FM, SR>),
-                               dim3((unsigned)ceil_div(p.lend, 256), (unsigned)p.n), dim3(256), compact_lds_cap<S>(),
-                               h.stream, p);
-            return;
-        }
+    if (h.compact) {
+        hipLaunchKernelGGL((collide_compact<S, BX, BY, BZ, OB, FM, SR>),
+                           dim3((unsigned)ceil_div(p.lend, 256), (unsigned)p.n), dim3(256), compact_lds_cap<S>(),
+                           h.stream, p);
+        return;
     }
     const RowMap rm = row_map(p.l);
     if constexpr (sizeof(S) == 2) {
@@ -292,7 +309,7 @@ void one_step(bgk_handle& h, hipEvent_t pre, hipEvent_t post)
 {
     if (h.cfg.boundary == BGK_BC_INFLOW) {
         hipLaunchKernelGGL(outflow_halo<S>, dim3(ceil_div((long)h.cfg.m * h.cfg.n, 256)), dim3(256), 0, h.stream,
-                           h.current<S>(), h.geom());
+                           h.current<S>(), h.geom(), h.compact, (typename Storage<S>::compute)h.u_inflow);
     }
     if (pre) (void)hipEventRecord(pre, h.stream);
     if (h.cfg.rounding && sizeof(S) == 2)
@@ -374,8 +391,6 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
                             ctr[ax], R + 4, size[ax], "xyz"[ax]);
     }
     if (c.layout < 0 || c.layout > 2) return fail(BGK_EINVAL, "layout %d", c.layout);
-    if (c.layout == 2 && c.boundary == BGK_BC_INFLOW)
-        return fail(BGK_EINVAL, "the compact layout needs folded x boundaries (not inflow/outflow)");
     h.cfg = c;
     h.esize = storage_bytes(c.storage);
     h.A = (int)(128 / h.esize);
@@ -385,7 +400,7 @@ int handle_setup(bgk_handle& h, const bgk_config& c)
     // faster (prefer_compact: enough of a row's last line unused, or, with
     // 32/64-bit storage, a lattice of 2^24 cells or more).
     const double waste = (double)(round_up(c.l, h.A) - c.l) / (double)c.l;
-    h.compact = c.boundary != BGK_BC_INFLOW && (c.layout == 2 || (c.layout == 0 && prefer_compact(c, waste)));
+    h.compact = c.layout == 2 || (c.layout == 0 && prefer_compact(c, waste));
     if (h.compact) {
         // cell (0, 1, k) of every population plane on a line start
         h.nx = c.l + 2;

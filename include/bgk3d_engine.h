@@ -65,7 +65,7 @@ typedef struct bgk_config {
                              reference's expression order (the plain F16 form uses a leaner
                              FP32 arithmetic); 2 MIXED2: storage BGK_F32, collision in FP64 */
     int layout;           /* 0 choose, 1 rows padded to 128-byte lines, 2 compact rows of l + 2
-                             cells (not with BGK_BC_INFLOW).  The default takes compact rows
+                             cells.  The default takes compact rows
                              where they measured faster: when padding would leave more than 3 %
                              of a row unused in its last line (9 % with 16-bit storage; a line
                              holds 32 values in FP32, 16 in FP64, 64 in 16-bit storage), and

@@ -58,10 +58,9 @@ def oracle_state(case, prec, steps):
 @pytest.mark.parametrize("name", list(CASES))
 def test_engine_bit_exact(gpu, name, storage, layout):
     """Both row layouts: padded to 128-byte lines (collide_fused) and
-    compact (collide_compact; not with inflow/outflow)."""
+    compact (collide_compact; with inflow/outflow its zeroed x halos are
+    rewritten before every collide)."""
     case = CASES[name]
-    if layout == 2 and case.boundary == "inflow":
-        pytest.skip("the compact layout needs folded x boundaries")
     steps = 37
     banner(f"engine {name} {storage} layout={layout} {case.l}x{case.m}x{case.n} steps={steps}")
     with Engine(case, storage, layout=layout) as eng:
@@ -84,10 +83,11 @@ def test_engine_bit_exact(gpu, name, storage, layout):
 
 
 @pytest.mark.parametrize("storage", ["f32", "f64"])
-def test_engine_draglift(gpu, storage):
+@pytest.mark.parametrize("layout", [1, 2])
+def test_engine_draglift(gpu, storage, layout):
     case = CASES["inflow"]
-    banner(f"engine draglift {storage}")
-    with Engine(case, storage) as eng:
+    banner(f"engine draglift {storage} layout={layout}")
+    with Engine(case, storage, layout=layout) as eng:
         eng.step(25)
         G = eng.populations()
         dl = eng.draglift()
@@ -209,8 +209,8 @@ def test_engine_rejects_bad_config(gpu):
         Engine(Case("channel", 8, 8, 8, wall_axis=2, force_axis=2), "f32")
     with pytest.raises(rt.BGKError):
         Engine(Case("channel", 8, 8, 8, wall_axis=1), "f32")
-    with pytest.raises(rt.BGKError):  # inflow reads its x halo: padded rows only
-        Engine(Case("inflow", 40, 24, 24, obstacle=True), "f32", layout=2)
+    with pytest.raises(rt.BGKError):
+        Engine(Case("inflow", 40, 24, 24, obstacle=True), "f32", layout=3)
 
 
 def solid_sphere_mask(l, m, n, centre, radius):
