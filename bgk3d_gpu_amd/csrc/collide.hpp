@@ -68,6 +68,9 @@ struct FusedArgs {
     // is not finite -- a non-finite density or velocity anywhere in the
     // cell's inputs makes it so.  Null: no check.
     uint32_t* bad;
+    // Inflow velocity: compact rows under the inflow boundary write both x
+    // faces from the halo lanes (compact_cells).
+    C uin;
 };
 
 // Raise the non-finite flag of p (if any) when the rest population v is not
@@ -77,6 +80,59 @@ __device__ __forceinline__ void flag_nonfinite(const FusedArgs<S>& p, typename S
 {
     if (p.bad && !__builtin_isfinite(v)) *gmem(p.bad) = 1u;
 }
+
+// Shifted equilibria of the inflow boundary's x faces
+// (SRC/bcond.inflow.F90:131-181, rho = 1, cte1 = 0).  Inflow face x = 0,
+// directions 1..5: u = (u_inflow, 0, 0), the same for every (j, k).
+template <typename S>
+__device__ __forceinline__ void inflow_face(typename Storage<S>::compute u_inflow, S (&v)[5])
+{
+    typedef typename Storage<S>::compute R;
+    const R p1 = W<R>::p1, p2 = W<R>::p2, rf = R(3), qf = R(1.5), three = R(3), crho = R(1), cte1 = R(0);
+    const R xj = u_inflow, yj = R(0), zj = R(0);
+    const R cvsq = xj * xj + yj * yj + zj * zj;
+    const R cx01 = rf * (xj - yj) + qf * (three * (xj - yj) * (xj - yj) - cvsq);
+    const R cx02 = rf * (xj - zj) + qf * (three * (xj - zj) * (xj - zj) - cvsq);
+    const R cx03 = rf * (xj + yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
+    const R cx04 = rf * (xj + zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
+    const R cx05 = rf * (xj) + qf * (three * (xj) * (xj)-cvsq);
+    v[0] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx01), 0u);
+    v[1] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx02), 0u);
+    v[2] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx03), 0u);
+    v[3] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx04), 0u);
+    v[4] = Rounder<S, 0>::cvt(crho * p1 * (cte1 + cx05), 0u);
+}
+
+// Outflow face x = l+1, directions 10..14: rho = 1 and the momentum of cell
+// (l, j, k), whose stored populations 1..18 are f[0..17].
+template <typename S>
+__device__ __forceinline__ void outflow_face(const typename Storage<S>::compute (&f)[19], S (&v)[5])
+{
+    typedef typename Storage<S>::compute R;
+    const R p1 = W<R>::p1, p2 = W<R>::p2, rf = R(3), qf = R(1.5), three = R(3), crho = R(1), rhoinv = R(1),
+            cte1 = R(0);
+    const R xj = +(f[0] + f[1] + f[2] + f[3] + f[4] - f[9] - f[10] - f[11] - f[12] - f[13]) * rhoinv;
+    const R yj = +(f[2] + f[6] + f[7] + f[8] + f[11] - f[0] - f[9] - f[15] - f[16] - f[17]) * rhoinv;
+    const R zj = +(f[3] + f[5] + f[6] + f[12] + f[17] - f[1] - f[8] - f[10] - f[14] - f[15]) * rhoinv;
+    const R cvsq = xj * xj + yj * yj + zj * zj;
+    const R cx10 = rf * (-xj - yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
+    const R cx11 = rf * (-xj - zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
+    const R cx12 = rf * (-xj + yj) + qf * (three * (xj - yj) * (xj - yj) - cvsq);
+    const R cx13 = rf * (-xj + zj) + qf * (three * (xj - zj) * (xj - zj) - cvsq);
+    const R cx14 = rf * (-xj) + qf * (three * (xj) * (xj)-cvsq);
+    v[0] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx10), 0u);
+    v[1] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx11), 0u);
+    v[2] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx12), 0u);
+    v[3] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx13), 0u);
+    v[4] = Rounder<S, 0>::cvt(crho * p1 * (cte1 + cx14), 0u);
+}
+
+// Whether the compact collide writes both x faces of the inflow boundary
+// itself: the forms that hold all 19 new populations before storing them
+// (every form but the streaming FP64 collide).  That one leaves the faces
+// to the step's outflow kernel.
+template <typename S, int FMODE>
+constexpr bool kFillsInflowFaces = !(std::is_same<typename Storage<S>::compute, double>::value && (FMODE & 4) == 0);
 
 // Base of population d of the lattice read (A) or written (B) this step.
 // CMP (compact): population d < 18 of a lattice at a[0] + d*pstride or
@@ -340,9 +396,16 @@ inline RowMap row_map_capped(int span, int maxw)
 // halo position that only completes a 128-byte line) stores zeros through
 // the same store instructions, so a wave that covers both kinds of lanes
 // issues its 19 stores once instead of branching into a second set.
-template <typename S, int FMODE, int SR, bool CMP>
+// FILL (compact rows, inflow boundary; kFillsInflowFaces): a halo lane with
+// face = 1 (x = 0) stores the inflow face's values in directions 1..5, with
+// face = 2 (x = l+1, its pull ran on cell l) the outflow face's values from
+// cell l's new populations as cell l stores them (its `cell` key is cell
+// l's, so stochastic rounding draws the same words) in directions 10..14,
+// zeros elsewhere -- what the outflow kernel would write before the next
+// collide.
+template <typename S, int FMODE, int SR, bool CMP, bool FILL = false>
 __device__ __forceinline__ void collide_store(const FusedArgs<S>& p, typename Storage<S>::compute (&x)[19], int i,
-                                              long row, uint32_t seed, uint32_t cell, bool live = true)
+                                              long row, uint32_t seed, uint32_t cell, bool live = true, int face = 0)
 {
     typedef typename Storage<S>::compute C;
     constexpr long ES = (long)sizeof(S);
@@ -358,6 +421,7 @@ __device__ __forceinline__ void collide_store(const FusedArgs<S>& p, typename St
         // without spilling).  The FP32 form keeps the array collide: with
         // its stores spread through the arithmetic, rows that end in a
         // partly interior line (n = 134 ... 182) ran 18-24 % slower.
+        static_assert(!FILL, "the streaming collide does not fill the inflow faces");
         bgk_collide_stream(x, p.omega, fx, fy, fz, p.cst, [&](auto dc, C v) {
             constexpr int d = decltype(dc)::value;
             S* dst;
@@ -381,6 +445,17 @@ __device__ __forceinline__ void collide_store(const FusedArgs<S>& p, typename St
     // over the cell's stores (deterministic: independent of scheduling)
     uint32_t r = 0;
     if (SR) r = hash32(seed ^ cell);
+    [[maybe_unused]] S hv[5];
+    if constexpr (FILL) {
+        if (face == 1) {
+            inflow_face<S>(p.uin, hv);
+        } else if (face == 2) {
+            C f[19];
+#pragma unroll
+            for (int d = 0; d < 19; ++d) f[d] = Storage<S>::widen(Rounder<S, SR>::cvt(o[d], SR ? sr_word(r, d) : 0u));
+            outflow_face<S>(f, hv);
+        }
+    }
 #pragma unroll
     for (int d = 0; d < 19; ++d) {
         S* dst;
@@ -388,7 +463,12 @@ __device__ __forceinline__ void collide_store(const FusedArgs<S>& p, typename St
             dst = (d < 18 ? p.b[d] : p.a[18]) + (row + i);
         else
             dst = reinterpret_cast<S*>(reinterpret_cast<char*>(pop_b<CMP>(p, d)) + row * ES + (uint64_t)vi);
-        store_nt(dst, Rounder<S, SR>::cvt(live ? o[d] : C(0), SR ? sr_word(r, d) : 0u));
+        S v = Rounder<S, SR>::cvt(live ? o[d] : C(0), SR ? sr_word(r, d) : 0u);
+        if constexpr (FILL) {
+            if (d < 5 && face == 1) v = hv[d];
+            if (d >= 9 && d < 14 && face == 2) v = hv[d - 9];
+        }
+        store_nt(dst, v);
     }
     flag_nonfinite(p, live ? o[18] : C(0));
 }
@@ -464,8 +544,10 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
 // (l+1, m) (p.lend positions): lanes on interior cells pull, collide and
 // store; lanes on the x halo cells and on the cells of row m+1 store zeros,
 // so every line is written whole.  A folded boundary never reads a halo
-// cell; the inflow boundary reads both x halos, and its outflow kernel
-// rewrites them in the source lattice before each collide.  The padded layout spends whole lines on a row's last
+// cell.  The inflow boundary reads both x halos: the forms of
+// kFillsInflowFaces store the faces' values from those lanes instead of
+// zeros (the lane on x = l+1 has run the collide of cell l), the others
+// leave them to the step's outflow kernel.  The padded layout spends whole lines on a row's last
 // few cells (a 129-cell FP32 row spans five lines for 4.03 lines of data);
 // this one wastes 2 cells per row, with rows that start anywhere in a line
 // (y-shifted pulls then span three lines per wave, the third shared with the
@@ -490,7 +572,9 @@ __device__ __forceinline__ void compact_cells(const FusedArgs<S>& p, int bx, int
     const long rowl = (long)p.nx * jl + p.nxy * (long)k;
     C x[19];
     pull_all<S, BX, BY, BZ, OBST, true, CMP>(p, il, jl, k, rowl, x);
-    collide_store<S, FMODE, SR, CMP>(p, x, i, row, seed, ref_cell(p, il, jl, k), live);
+    constexpr bool FILL = BX == AX_HALO && kFillsInflowFaces<S, FMODE>;
+    const int face = (FILL && j <= p.m) ? (i == 0 ? 1 : (i == p.l + 1 ? 2 : 0)) : 0;
+    collide_store<S, FMODE, SR, CMP, FILL>(p, x, i, row, seed, ref_cell(p, il, jl, k), live, face);
 }
 
 // Waves per SIMD the compiler is asked to fit (1: no request).  FP64

@@ -61,31 +61,9 @@ namespace {
 // kernels specific to the engine
 // ---------------------------------------------------------------------------
 
-// Shifted equilibrium of the inflow face x = 0, directions 1..5, for
-// rho = 1, u = (u_inflow, 0, 0) (SRC/bcond.inflow.F90:165-181, cte1 = 0):
-// the same values for every (j, k).
-template <typename S>
-__device__ __forceinline__ void inflow_values(typename Storage<S>::compute u_inflow, S (&v)[5])
-{
-    typedef typename Storage<S>::compute R;
-    const R p1 = W<R>::p1, p2 = W<R>::p2, rf = R(3), qf = R(1.5), three = R(3), crho = R(1), cte1 = R(0);
-    const R xj = u_inflow, yj = R(0), zj = R(0);
-    const R cvsq = xj * xj + yj * yj + zj * zj;
-    const R cx01 = rf * (xj - yj) + qf * (three * (xj - yj) * (xj - yj) - cvsq);
-    const R cx02 = rf * (xj - zj) + qf * (three * (xj - zj) * (xj - zj) - cvsq);
-    const R cx03 = rf * (xj + yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
-    const R cx04 = rf * (xj + zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
-    const R cx05 = rf * (xj) + qf * (three * (xj) * (xj)-cvsq);
-    v[0] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx01), 0u);
-    v[1] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx02), 0u);
-    v[2] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx03), 0u);
-    v[3] = Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx04), 0u);
-    v[4] = Rounder<S, 0>::cvt(crho * p1 * (cte1 + cx05), 0u);
-}
-
-// Inflow halo x = 0, directions 1..5, every (j,k) incl. halo rows.
-// Constant in time: with rows padded to whole lines the collide never
-// writes a halo cell, and this runs once per lattice.
+// Inflow halo x = 0, directions 1..5, every (j,k) incl. halo rows, the
+// values of inflow_face.  Constant in time: with rows padded to whole lines
+// the collide never writes a halo cell, and this runs once per lattice.
 template <typename S>
 __global__ void inflow_halo(Lattice19<S> a, Geometry g, typename Storage<S>::compute u_inflow)
 {
@@ -94,17 +72,16 @@ __global__ void inflow_halo(Lattice19<S> a, Geometry g, typename Storage<S>::com
     if (t >= nj * (g.n + 2)) return;
     const int j = t % nj, k = t / nj;
     S v[5];
-    inflow_values<S>(u_inflow, v);
+    inflow_face<S>(u_inflow, v);
     const long s = gidx(g, 0, j, k);
     for (int d = 0; d < 5; ++d) store_value(a.p[d] + s, v[d]);
 }
 
-// Outflow halo x = l+1, directions 10..14, interior (j,k): shifted
-// equilibrium with rho = 1 and the momentum of cell (l,j,k)
-// (SRC/bcond.inflow.F90:131-162).  The fused pull wraps j and k before
-// reading this halo, which reproduces the reference's y/z periodic copies of
-// the halo rows.  With compact rows the previous collide stored zeros over
-// both x halos of this lattice (its lanes write whole lines), so `refill`
+// Outflow halo x = l+1, directions 10..14, interior (j,k): outflow_face of
+// cell (l,j,k).  The fused pull wraps j and k before reading this halo,
+// which reproduces the reference's y/z periodic copies of the halo rows.
+// Compact rows: the previous collide stored zeros over both x halos of this
+// lattice unless it wrote the faces itself (kFillsInflowFaces), so `refill`
 // also writes the inflow face x = 0 of the rows the pull reads (interior j,
 // k), with the values inflow_halo writes.
 template <typename S>
@@ -113,31 +90,18 @@ __global__ void outflow_halo(Lattice19<S> a, Geometry g, bool refill, typename S
     typedef typename Storage<S>::compute R;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.m * g.n) return;
-    const int j = 1 + t % g.m, k = 1 + t / g.m, l = g.l;
-    const R p1 = W<R>::p1, p2 = W<R>::p2, rf = R(3), qf = R(1.5), three = R(3), crho = R(1), rhoinv = R(1),
-            cte1 = R(0);
-    const long si = gidx(g, l, j, k);
+    const int j = 1 + t % g.m, k = 1 + t / g.m;
+    const long si = gidx(g, g.l, j, k);
     R f[19];
 #pragma unroll
     for (int d = 0; d < 18; ++d) f[d] = Storage<S>::load(a.p[d] + si);
-    const R xj = +(f[0] + f[1] + f[2] + f[3] + f[4] - f[9] - f[10] - f[11] - f[12] - f[13]) * rhoinv;
-    const R yj = +(f[2] + f[6] + f[7] + f[8] + f[11] - f[0] - f[9] - f[15] - f[16] - f[17]) * rhoinv;
-    const R zj = +(f[3] + f[5] + f[6] + f[12] + f[17] - f[1] - f[8] - f[10] - f[14] - f[15]) * rhoinv;
-    const R cvsq = xj * xj + yj * yj + zj * zj;
-    const R cx10 = rf * (-xj - yj) + qf * (three * (xj + yj) * (xj + yj) - cvsq);
-    const R cx11 = rf * (-xj - zj) + qf * (three * (xj + zj) * (xj + zj) - cvsq);
-    const R cx12 = rf * (-xj + yj) + qf * (three * (xj - yj) * (xj - yj) - cvsq);
-    const R cx13 = rf * (-xj + zj) + qf * (three * (xj - zj) * (xj - zj) - cvsq);
-    const R cx14 = rf * (-xj) + qf * (three * (xj) * (xj)-cvsq);
-    const long so = gidx(g, l + 1, j, k);
-    store_value(a.p[9] + so, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx10), 0u));
-    store_value(a.p[10] + so, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx11), 0u));
-    store_value(a.p[11] + so, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx12), 0u));
-    store_value(a.p[12] + so, Rounder<S, 0>::cvt(crho * p2 * (cte1 + cx13), 0u));
-    store_value(a.p[13] + so, Rounder<S, 0>::cvt(crho * p1 * (cte1 + cx14), 0u));
+    f[18] = R(0);  // not used
+    S v[5];
+    outflow_face<S>(f, v);
+    const long so = gidx(g, g.l + 1, j, k);
+    for (int d = 0; d < 5; ++d) store_value(a.p[9 + d] + so, v[d]);
     if (refill) {
-        S v[5];
-        inflow_values<S>(u_inflow, v);
+        inflow_face<S>(u_inflow, v);
         const long s0 = gidx(g, 0, j, k);
         for (int d = 0; d < 5; ++d) store_value(a.p[d] + s0, v[d]);
     }
@@ -303,11 +267,23 @@ void dispatch_fused(bgk_handle& h)
         dispatch_fused_les<S, SR, 0>(h);
 }
 
-// queue the boundary work and the fused collide of one step, then swap
+// Whether this handle's compact collide writes the inflow boundary's x
+// faces itself (kFillsInflowFaces of the form it launches).
 template <typename S>
-void one_step(bgk_handle& h, hipEvent_t pre, hipEvent_t post)
+bool collide_fills_faces(const bgk_handle& h)
 {
-    if (h.cfg.boundary == BGK_BC_INFLOW) {
+    if (!h.compact || h.cfg.boundary != BGK_BC_INFLOW) return false;
+    return h.cfg.les ? kFillsInflowFaces<S, 4> : kFillsInflowFaces<S, 0>;
+}
+
+// queue the boundary work and the fused collide of one step, then swap.
+// `first`: the first step of a stepping call, after which the lattice may
+// have been set or restored -- the outflow kernel then runs whatever the
+// collide writes.
+template <typename S>
+void one_step(bgk_handle& h, hipEvent_t pre, hipEvent_t post, bool first)
+{
+    if (h.cfg.boundary == BGK_BC_INFLOW && (first || !collide_fills_faces<S>(h))) {
         hipLaunchKernelGGL(outflow_halo<S>, dim3(ceil_div((long)h.cfg.m * h.cfg.n, 256)), dim3(256), 0, h.stream,
                            h.current<S>(), h.geom(), h.compact, (typename Storage<S>::compute)h.u_inflow);
     }
@@ -583,7 +559,7 @@ int bgk_step(bgk_handle* h, int nsteps)
     if (!h || nsteps < 0) return fail(BGK_EINVAL, "bad arguments");
     ENG_HIP(hipSetDevice(h->cfg.device));
     return by_storage(h->cfg, [&](auto tag) {
-        for (int s = 0; s < nsteps; ++s) one_step<decltype(tag)>(*h, nullptr, nullptr);
+        for (int s = 0; s < nsteps; ++s) one_step<decltype(tag)>(*h, nullptr, nullptr, s == 0);
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : fail(BGK_EHIP, "step launch: %s", hipGetErrorString(e));
     });
@@ -611,7 +587,7 @@ int bgk_time_steps(bgk_handle* h, int nsteps, double* total_ms, double* collide_
     ENG_HIP(hipEventRecord(e0, h->stream));
     int rc = by_storage(h->cfg, [&](auto tag) {
         for (int s = 0; s < nsteps; ++s)
-            one_step<decltype(tag)>(*h, h->events[2 + 2 * s], h->events[3 + 2 * s]);
+            one_step<decltype(tag)>(*h, h->events[2 + 2 * s], h->events[3 + 2 * s], s == 0);
         return 0;
     });
     if (rc) return rc;

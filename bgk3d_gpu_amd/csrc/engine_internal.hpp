@@ -197,6 +197,7 @@ FusedArgs<S> fused_args(const bgk_handle& h, int cur)
         p.lend = h.cfg.boundary == BGK_BC_INFLOW ? 0 : (int)round_up(h.cfg.l, h.A);
     p.seed = step_seed(h.cfg.seed, h.steps);
     p.pstride = h.plane;
+    p.uin = (C)h.u_inflow;
     return p;
 }
 
