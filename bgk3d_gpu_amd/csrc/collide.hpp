@@ -600,12 +600,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FusedWaves<
 // uses none; 32 KiB per 256-thread block holds the FP32 form to 5 waves per
 // SIMD (a CU has 160 KiB) instead of 8: fewer streams in flight over rows
 // that do not start on a line ran 8 % faster at 256^3 (463 against 502 us)
-// and 4-5 % at n = 129 ... 192.  The 16-bit forms ran slower so capped (259
-// against 245 us at 256^3) and keep their natural occupancy.
+// and 4-5 % at n = 129 ... 192.  MIXED2 (FP32 storage, FP64 collide, 6
+// waves by register use) likewise: 425 against 436 us at 256^3.  FP64
+// storage (883 against 871-878 us, 40 KiB no better) and the 16-bit forms
+// (259 against 245 us) ran slower so capped and keep their natural
+// occupancy.
 template <typename S>
 constexpr unsigned compact_lds_cap()
 {
-    return std::is_same<S, float>::value ? (32u << 10) : 0u;
+    return (std::is_same<S, float>::value || std::is_same<S, mix32_t>::value) ? (32u << 10) : 0u;
 }
 
 template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
