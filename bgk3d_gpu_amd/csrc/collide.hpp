@@ -547,11 +547,12 @@ __device__ __forceinline__ void fused_rows(const FusedArgs<S>& p, const RowMap& 
 // cell.  The inflow boundary reads both x halos: the forms of
 // kFillsInflowFaces store the faces' values from those lanes instead of
 // zeros (the lane on x = l+1 has run the collide of cell l), the others
-// leave them to the step's outflow kernel.  The padded layout spends whole lines on a row's last
-// few cells (a 129-cell FP32 row spans five lines for 4.03 lines of data);
-// this one wastes 2 cells per row, with rows that start anywhere in a line
-// (y-shifted pulls then span three lines per wave, the third shared with the
-// neighbouring wave through L2).  bz = k - 1, bx = position block of 256.
+// leave them to the step's outflow kernel.  The padded layout spends whole
+// lines on a row's last few cells (a 129-cell FP32 row spans five lines for
+// 4.03 lines of data); this one wastes 2 cells per row, with rows that start
+// anywhere in a line (y-shifted pulls then span three lines per wave, the
+// third shared with the neighbouring wave through L2).  bz = k - 1, bx =
+// position block of 256.
 template <typename S, int BX, int BY, int BZ, bool OBST, int FMODE, int SR>
 __device__ __forceinline__ void compact_cells(const FusedArgs<S>& p, int bx, int bz, uint32_t seed)
 {
