@@ -137,13 +137,25 @@ def load_traffic(workload, kernel):
     return None
 
 
-def bench_v1(case, prec, steps, warmup):
+def bench_v1(case, prec, steps, warmup, whole_lines=False):
     """Reference-compatible ABI driven the reference's way (one
     bcond_periodic_gpu + col_mc_gpu per step, caller-owned arrays); total
-    milliseconds of the timed steps."""
+    milliseconds of the timed steps.  whole_lines: with BGK3D_WHOLE_LINES=1,
+    which lets col_mc_gpu store zeros into the halo and padding cells between
+    interior rows so that it writes whole 128-byte lines."""
     from bgk3d_gpu_amd.studies.reference_abi import time_steps
 
-    return time_steps(case, prec, steps, warmup)
+    if not whole_lines:
+        return time_steps(case, prec, steps, warmup)
+    old = os.environ.get("BGK3D_WHOLE_LINES")
+    os.environ["BGK3D_WHOLE_LINES"] = "1"
+    try:
+        return time_steps(case, prec, steps, warmup)
+    finally:
+        if old is None:
+            del os.environ["BGK3D_WHOLE_LINES"]
+        else:
+            os.environ["BGK3D_WHOLE_LINES"] = old
 
 
 def bench_engine(case, storage, steps, warmup, want_ke=False, rounding=None, **kw):
@@ -331,6 +343,12 @@ def main():
                                       "ms_per_step": round(t / fsteps, 4),
                                       "note": "reference layout (nx=l+2), halo kernels per step"}
         log(f"v1: {forms['v1_abi_' + storage]}")
+        t = bench_v1(case, args.prec, fsteps, 5, whole_lines=True)
+        forms["v1_abi_" + storage + "_whole_lines"] = {
+            "mlups": round(case.cells * fsteps / (t * 1e-3) / 1e6, 1), "ms_per_step": round(t / fsteps, 4),
+            "note": "as v1_abi with BGK3D_WHOLE_LINES=1: col_mc_gpu also stores zeros into the halo and padding "
+                    "cells between interior rows (values no routine reads), so it writes whole lines"}
+        log(f"v1 whole lines: {forms['v1_abi_' + storage + '_whole_lines']}")
         if "f16" in forms:
             forms["f16_over_" + storage] = round(forms["f16"]["mlups"] / mlups, 3)
         if not args.no_sphere:

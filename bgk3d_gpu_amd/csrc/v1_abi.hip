@@ -22,6 +22,7 @@
 #include "../../include/bgk3d_gpu.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 
 #ifdef BGK_DOUBLE
@@ -116,6 +117,20 @@ void launch_face(K kern, long count, const char* name, Args... args)
 // per SIMD gained half of that, 4 the same as 5.
 constexpr unsigned kFlatLdsCap = 32u << 10;
 
+// BGK3D_WHOLE_LINES=1 in the environment lets col_mc_gpu store zeros into
+// the halo and padding cells between the interior rows of the arrays it
+// writes (b01..b18 and a19), so that its flat mapping writes whole 128-byte
+// lines (collide_flat, WL).  The reference's routines never read those values:
+// each bcond_*_gpu call rewrites every halo value the next collide pulls, and
+// the diagnostics read interior cells.  Only a raw dump of whole arrays
+// (halo included) sees the difference.  Unset: the caller's halo and padding
+// are left untouched, as the reference's col_MC leaves them.
+bool whole_lines()
+{
+    const char* v = std::getenv("BGK3D_WHOLE_LINES");
+    return v && v[0] == '1' && v[1] == '\0';
+}
+
 // The fused kernel in halo mode: every pull reads the caller's halo, which
 // the bcond_*_gpu entry points filled.  Waves are aligned to 128-byte lines
 // from the output addresses, whatever padding (ipad) the caller chose.
@@ -162,8 +177,13 @@ void launch_col(const Lattice19<real_t>& A, const Lattice19<real_t>& B, const Ge
     const long flat_cells = (long)g.nx * (g.m - 1) + g.l + p.align - 1;
     if (g.nxy < (1L << 31) && ceil_div(flat_cells, 64L) < row_waves) {
         const dim3 grid((unsigned)ceil_div(flat_cells, 256L), (unsigned)g.n);
-        if (g.nx % p.align == 0 && g.nxy % p.align == 0)
+        const bool nt0 = g.nx % p.align == 0 && g.nxy % p.align == 0, wl = whole_lines();
+        if (nt0 && wl)
+            hipLaunchKernelGGL((collide_flat<real_t, FM, true, true>), grid, dim3(256), kFlatLdsCap, abi_stream(), p);
+        else if (nt0)
             hipLaunchKernelGGL((collide_flat<real_t, FM, true>), grid, dim3(256), kFlatLdsCap, abi_stream(), p);
+        else if (wl)
+            hipLaunchKernelGGL((collide_flat<real_t, FM, false, true>), grid, dim3(256), kFlatLdsCap, abi_stream(), p);
         else
             hipLaunchKernelGGL((collide_flat<real_t, FM, false>), grid, dim3(256), kFlatLdsCap, abi_stream(), p);
         report_launch("collide_flat");
