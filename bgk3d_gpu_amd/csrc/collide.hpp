@@ -604,10 +604,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FusedWaves<
 // uses none; 32 KiB per 256-thread block holds the FP32 form to 5 waves per
 // SIMD (a CU has 160 KiB) instead of 8: fewer streams in flight over rows
 // that do not start on a line ran 8 % faster at 256^3 (463 against 502 us)
-// and 4-5 % at n = 129 ... 192.  MIXED2 (FP32 storage, FP64 collide, 6
-// waves by register use) likewise: 425 against 436 us at 256^3.  FP64
-// storage (883 against 871-878 us, 40 KiB no better) and the 16-bit forms
-// (259 against 245 us) ran slower so capped and keep their natural
+// and 4-5 % at n = 129 ... 192; 6 waves (26 KiB) took 426-428 against
+// 420-423 us, 4 waves (40 KiB) the same as 5.  MIXED2 (FP32 storage, FP64
+// collide, 6 waves by register use) likewise: 425 against 436 us at 256^3.
+// FP64 storage (883 against 871-878 us, 40 KiB no better) and the 16-bit
+// forms (259 against 245 us) ran slower so capped and keep their natural
 // occupancy.
 template <typename S>
 constexpr unsigned compact_lds_cap()
