@@ -201,19 +201,20 @@ def test_layouts_agree_for_16_bit_storage(gpu, storage, rounding):
     assert np.any(out[0] != 0)
 
 
-@pytest.mark.parametrize("storage,rounding", [("f32", 0), ("f64", 0), ("f16", 0), ("f16", 1), ("bf16", 0)])
+@pytest.mark.parametrize("storage,rounding,mixed", [("f32", 0, 0), ("f64", 0, 0), ("f16", 0, 0), ("f16", 1, 0),
+                                                    ("bf16", 0, 0), ("f16", 0, 1), ("f32", 0, 2)])
 @pytest.mark.parametrize("name", ["inflow_long", "inflow_les"])
-def test_inflow_layouts_agree(gpu, name, storage, rounding):
+def test_inflow_layouts_agree(gpu, name, storage, rounding, mixed):
     """Inflow boundary: padded rows (the outflow kernel before every collide)
     and compact rows (both x faces written by the collide's halo lanes in
     the forms that hold all 19 new populations, by the outflow kernel for
     the streaming FP64 collide) give the same bits, stepped in one call or
     in several (each call starts with the outflow kernel)."""
     case = CASES[name]
-    banner(f"engine inflow layouts {name} {storage} rounding={rounding}")
+    banner(f"engine inflow layouts {name} {storage} rounding={rounding} mixed={mixed}")
     out = []
     for layout, chunks in ((1, [40]), (2, [40]), (2, [1, 7, 32])):
-        with Engine(case, storage, rounding=rounding, layout=layout, seed=5) as eng:
+        with Engine(case, storage, rounding=rounding, layout=layout, seed=5, mixed=mixed) as eng:
             for c in chunks:
                 eng.step(c)
             out.append(eng.populations())
